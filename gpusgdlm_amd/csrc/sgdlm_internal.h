@@ -1,0 +1,98 @@
+/*
+ * sgdlm_internal.h -- declarations shared by the HIP translation units of
+ * libsgdlm_hip.so (kernel launchers and the device-side problem layout).
+ */
+#ifndef SGDLM_INTERNAL_H
+#define SGDLM_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sgdlm.h"
+
+namespace sgdlm {
+
+/* Simultaneous-parent structure in the form the kernels use.  The packed
+ * draw layout stores only the p_j meaningful coefficients of each series:
+ * theta_packed[i * P + off[j] + k], k < p_j. */
+struct Structure {
+  int m = 0, max_p = 0;
+  int M = 0;            /* m rounded up to a multiple of 16 (LU dimension)   */
+  long P = 0;           /* sum_j p_j                                          */
+  int n_gam = 0;        /* number of simultaneous coefficients (Gamma nnz)   */
+  const unsigned *p = nullptr;   /* device, m                                */
+  const unsigned *sp = nullptr;  /* device, m * max_p                        */
+  const int *off = nullptr;      /* device, m + 1 prefix sums of p           */
+  /* Gamma entries sorted by (row, col): A[row][col] = -theta_packed[idx]   */
+  const int *g_row = nullptr, *g_col = nullptr, *g_idx = nullptr;
+  const int *g_rowptr = nullptr; /* device, m + 1 (CSR over rows)           */
+};
+
+/* ---- filter (k_filter.hip) ---- */
+void launch_posterior(const Structure &S, double *m_t, double *C_t,
+                      double *n_t, double *s_t, const double *y,
+                      const double *F, hipStream_t st);
+void launch_prior(const Structure &S, double *m_t, double *C_t, double *n_t,
+                  double *W_t, const double *beta, const double *delta,
+                  const double *G, hipStream_t st);
+
+/* ---- draws (k_draws.hip) ---- */
+void launch_cholesky(const Structure &S, const double *C, double *U,
+                     hipStream_t st);
+void launch_draw(const Structure &S, long N, const double *m_t,
+                 const double *U, const double *n_t, const double *s_t,
+                 uint64_t seed, uint32_t epoch, double *lam, double *theta,
+                 hipStream_t st);
+void launch_forecast_rhs(const Structure &S, long N, const double *x,
+                         const double *m_t, const double *theta,
+                         const double *lam, uint64_t seed, uint32_t epoch,
+                         double *rhs, hipStream_t st);
+void launch_evolve(const Structure &S, long N, const double *U_W,
+                   const double *r_t, const double *c_t, const double *beta,
+                   const double *G, const double *m_t, uint64_t seed,
+                   uint32_t epoch, double *lam, double *theta,
+                   hipStream_t st);
+void launch_unpack_theta(const Structure &S, long N, const double *theta,
+                         const double *m_t, double *theta_full,
+                         hipStream_t st);
+
+/* ---- LU (k_lu.hip) ---- */
+struct LuPlan {
+  int prec = SGDLM_PREC_FP64; /* element type / update path                */
+  int nb = 0;                 /* panel width                                */
+  int grid = 0;               /* persistent workgroups                      */
+  size_t lds = 0;             /* dynamic LDS bytes                          */
+  size_t work_bytes = 0;      /* global workspace bytes (grid * M * M * T)  */
+};
+int lu_plan(const Structure &S, int prec, int solve, int n_cu, LuPlan *plan);
+int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
+                     const double *theta, void *work, double *logdet,
+                     hipStream_t st);
+int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
+                    const double *theta, void *work, const double *rhs,
+                    double *y, int refine, hipStream_t st);
+
+/* ---- VB (k_vb.hip) ---- */
+struct VbBuffers {
+  double *w = nullptr;        /* N normalised weights                      */
+  double *stats = nullptr;    /* [0] max logdet, [1] sum exp(l - max),
+                                 [2] sum w^2, [3] number of zero weights     */
+  double *part1 = nullptr;    /* chunks * (m * (2 + max_p)) partial sums    */
+  double *part2 = nullptr;    /* chunks * (m * max_p^2) partial sums        */
+  double *mean_lambda = nullptr, *mean_log_lambda = nullptr; /* m each      */
+  double *mean_m = nullptr;   /* m * max_p                                   */
+  double *mean_C = nullptr;   /* m * max_p^2                                 */
+  double *mean_n = nullptr, *mean_s = nullptr, *mean_Q = nullptr; /* m      */
+  int *newton_iter = nullptr; /* m                                           */
+  int chunks = 0;
+};
+int vb_chunks(long N);
+void launch_weights(long N, const double *logdet, double *w, double *stats,
+                    hipStream_t st);
+void launch_moments(const Structure &S, long N, const double *lam,
+                    const double *theta, const double *n_t, const double *s_t,
+                    VbBuffers &B, hipStream_t st);
+
+} // namespace sgdlm
+
+#endif

@@ -1,0 +1,163 @@
+/*
+ * sgdlm.h -- C-ABI of the MI355X-native SGDLM engine (libsgdlm_hip.so).
+ *
+ * One entry point per method of the reference's C++ plugin interface
+ * SGDLM::HostWrapper<double> (rSGDLM/src/SGDLM/HostWrapper.hpp:13-99), which
+ * the Rcpp module gpuSGDLM/SGDLM calls (rSGDLM/src/SGDLM/RcppWrapper.cpp).
+ * A C++ implementation of that interface over these functions is in
+ * include/sgdlm/HostWrapper.hpp + gpusgdlm_amd/csrc/host_wrapper.cpp.
+ *
+ * Conventions
+ *   - Every function returns an int status (SGDLM_OK on success).  The
+ *     reference logs and returns early on errors (logger.hpp:160-165); here
+ *     the reason is kept in sgdlm_last_error().
+ *   - Host buffers are caller-owned, in the reference's column-major R
+ *     layouts flattened to double* / unsigned int*:
+ *       p[m], sp[j*max_p + k], beta[m], delta[j*max_p^2 + r + c*max_p],
+ *       m_t[j*max_p + k], C_t[j*max_p^2 + r + c*max_p], n_t[m], s_t[m],
+ *       y_t[m], F_t[j*max_p + k], G[j*max_p^2 + r + c*max_p],
+ *       IS weights[nsim], forecast y[i*m + j], lambdas[i*m + j],
+ *       thetas[(i*m + j)*max_p + k].
+ *     NULL output pointers are skipped, as in the reference getters.
+ *   - Calls are synchronous.  A context is thread-compatible (one thread at
+ *     a time), not thread-safe.
+ *   - One GPU per context (the reference's no_gpus is clamped to 1).
+ */
+#ifndef SGDLM_H
+#define SGDLM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGDLM_OK 0
+#define SGDLM_ERR_ARG 1     /* invalid argument or dimensions              */
+#define SGDLM_ERR_STATE 2   /* call not allowed in the current state       */
+#define SGDLM_ERR_DEVICE 3  /* HIP runtime error                           */
+#define SGDLM_ERR_ALLOC 4   /* device allocation failed                    */
+#define SGDLM_ERR_NUMERIC 5 /* e.g. every importance weight is zero        */
+
+/* Arithmetic of the dense det / solve work (LU of I - Gamma). */
+#define SGDLM_PREC_FP64 0   /* FP64 LU (MFMA f64 trailing updates)        */
+#define SGDLM_PREC_FP32 1   /* FP32 LU on MFMA, FP64 log-det sums          */
+#define SGDLM_PREC_BF16X3 2 /* BF16x3 split-precision MFMA trailing update */
+
+typedef struct sgdlm_ctx sgdlm_ctx;
+
+/* HostWrapperFactory<double>::create(no_gpus) -> HostWrapperImpl ctor,
+ * HostWrapperFactory.cu:5-11, HostWrapperImpl.cu:11-74 (device, stream,
+ * generator seeded 1234). */
+int sgdlm_create(sgdlm_ctx **ctx, size_t no_gpus);
+/* ~HostWrapperImpl, HostWrapperImpl.cu:76-104. */
+int sgdlm_destroy(sgdlm_ctx *ctx);
+/* Text of the last error of this context (empty if none). */
+const char *sgdlm_last_error(const sgdlm_ctx *ctx);
+
+/* HostWrapper.hpp:17-21 getNoSeries / getMaxP /
+ * getEvolutionMatrixConfiguration. */
+int sgdlm_get_no_series(const sgdlm_ctx *ctx, size_t *m);
+int sgdlm_get_max_p(const sgdlm_ctx *ctx, size_t *max_p);
+int sgdlm_get_evolution_matrix_configuration(const sgdlm_ctx *ctx,
+                                             int *use_state_evolution_matrix);
+/* HostWrapper.hpp:23-32 initMemory / clearMemory (HostWrapperImpl.cu:129-300). */
+int sgdlm_init_memory(sgdlm_ctx *ctx, size_t m, size_t max_p);
+int sgdlm_clear_memory(sgdlm_ctx *ctx);
+/* HostWrapper.hpp:34 manageEvoMemory (HostWrapperImpl.cu:230-279). */
+int sgdlm_manage_evo_memory(sgdlm_ctx *ctx, int use_state_evolution_matrix);
+/* HostWrapper.hpp:36-42 getNSim / getNSimBatch / initSimMemory /
+ * clearSimMemory (HostWrapperImpl.cu:302-537).  nsim_batch is kept as the
+ * reference's memory hint; results never depend on it. */
+int sgdlm_get_nsim(const sgdlm_ctx *ctx, size_t *nsim);
+int sgdlm_get_nsim_batch(const sgdlm_ctx *ctx, size_t *nsim_batch);
+int sgdlm_init_sim_memory(sgdlm_ctx *ctx, size_t nsim, size_t nsim_batch);
+int sgdlm_clear_sim_memory(sgdlm_ctx *ctx);
+/* HostWrapper.hpp:44 initForecastPriors (HostWrapperImpl.cu:539-587). */
+int sgdlm_init_forecast_priors(sgdlm_ctx *ctx);
+/* HostWrapper.hpp:46-48 isPrior() / isPrior(bool). */
+int sgdlm_get_is_prior(const sgdlm_ctx *ctx, int *is_prior);
+int sgdlm_set_is_prior(sgdlm_ctx *ctx, int is_prior);
+/* HostWrapper.hpp:50-58 getParameters / setParameters. */
+int sgdlm_get_parameters(const sgdlm_ctx *ctx, double *m_t, double *C_t,
+                         double *n_t, double *s_t);
+int sgdlm_set_parameters(sgdlm_ctx *ctx, const double *m_t, const double *C_t,
+                         const double *n_t, const double *s_t);
+/* HostWrapper.hpp:60-64 getDiscountFactors / setDiscountFactors. */
+int sgdlm_get_discount_factors(const sgdlm_ctx *ctx, double *beta,
+                               double *delta);
+int sgdlm_set_discount_factors(sgdlm_ctx *ctx, const double *beta,
+                               const double *delta);
+/* HostWrapper.hpp:66-68 getEvolutionMatrix / setEvolutionMatrix. */
+int sgdlm_get_evolution_matrix(const sgdlm_ctx *ctx, double *G_t);
+int sgdlm_set_evolution_matrix(sgdlm_ctx *ctx, const double *G_t);
+/* HostWrapper.hpp:70-74 getParentalSets / setParentalSets.  sp values
+ * v < m*m are positions row + col*m of Gamma (row = child); anything else
+ * (R's NA -> 2^31) marks an external regressor.  Rejected with SGDLM_ERR_ARG
+ * (the reference's result would be a write race or undefined): a diagonal
+ * position, a position used twice, a simultaneous position at k >= p_j,
+ * p_j > max_p. */
+int sgdlm_get_parental_sets(const sgdlm_ctx *ctx, unsigned int *p,
+                            unsigned int *sp);
+int sgdlm_set_parental_sets(sgdlm_ctx *ctx, const unsigned int *p,
+                            const unsigned int *sp);
+/* HostWrapper.hpp:76-79 computePrior(bool) / computePrior(G, bool)
+ * (HostWrapperImpl.cu:799-877; SGDLM.cu:115-179).  G_tp1 may be NULL. */
+int sgdlm_compute_prior(sgdlm_ctx *ctx, const double *G_tp1,
+                        int evolve_forecast_variables);
+/* HostWrapper.hpp:81-83 evolveForecastSamples() / (G)
+ * (HostWrapperImpl.cu:879-946; SGDLM.cu:679-762).  G_tp1 may be NULL. */
+int sgdlm_evolve_forecast_samples(sgdlm_ctx *ctx, const double *G_tp1);
+/* HostWrapper.hpp:85-89 computeForecast (HostWrapperImpl.cu:948-1021;
+ * SGDLM.cu:505-621).  y: nsim x m; lambdas, thetas may be NULL. */
+int sgdlm_compute_forecast(sgdlm_ctx *ctx, double *y_tp1, const double *x_tp1,
+                           double *lambdas, double *thetas,
+                           int use_existing_lambdas_and_thetas);
+/* HostWrapper.hpp:91-92 computePosterior (HostWrapperImpl.cu:1023-1057;
+ * SGDLM.cu:10-113). */
+int sgdlm_compute_posterior(sgdlm_ctx *ctx, const double *y_t,
+                            const double *F_t);
+/* HostWrapper.hpp:94-99 computeVBPosterior (HostWrapperImpl.cu:1059-1228;
+ * SGDLM.cu:181-503).  With every output NULL the results stay on the device
+ * (no PCIe copy); sgdlm_get_vb_result fetches them later. */
+int sgdlm_compute_vb_posterior(sgdlm_ctx *ctx, double *mean_m_t,
+                               double *mean_C_t, double *mean_n_t,
+                               double *mean_s_t, double *IS_weights,
+                               double *sum_det_weights);
+
+/* ---- Extensions (no counterpart in the reference) ---------------------- */
+
+/* LU arithmetic for the VB log-determinants and forecast solves
+ * (SGDLM_PREC_*; default SGDLM_PREC_FP64). */
+int sgdlm_set_precision(sgdlm_ctx *ctx, int precision);
+int sgdlm_get_precision(const sgdlm_ctx *ctx, int *precision);
+/* Philox state: key = seed (default 1234, the reference's cuRAND seed at
+ * HostWrapperImpl.cu:63-64) and the epoch counter advanced by every call
+ * that draws random numbers.  Checkpointable. */
+int sgdlm_set_rng_state(sgdlm_ctx *ctx, uint64_t seed, uint32_t epoch);
+int sgdlm_get_rng_state(const sgdlm_ctx *ctx, uint64_t *seed,
+                        uint32_t *epoch);
+/* Results of the last VB call left on the device: same layouts as
+ * sgdlm_compute_vb_posterior, plus log_weights (nsim, log of the
+ * unnormalised weight, -inf for a zero weight), ess = 1 / sum w^2, and
+ * log_sum_det = log(sum_det_weights).  Any pointer may be NULL. */
+int sgdlm_get_vb_result(const sgdlm_ctx *ctx, double *mean_m_t,
+                        double *mean_C_t, double *mean_n_t, double *mean_s_t,
+                        double *IS_weights, double *log_weights, double *ess,
+                        double *log_sum_det);
+/* Per-stage device timing with HIP events recorded on the context's stream
+ * around each launch (enable = 1 resets the counters).  stage: "lu" (VB
+ * log-det LU), "draw", "weights", "moments", "forecast_lu", "filter". */
+int sgdlm_set_timing(sgdlm_ctx *ctx, int enable);
+int sgdlm_get_timing(const sgdlm_ctx *ctx, const char *stage, double *ms,
+                     long *launches);
+/* Number of workgroups / LDS bytes / panel width chosen for the VB LU. */
+int sgdlm_get_lu_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
+                        int *nb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
