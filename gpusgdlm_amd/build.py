@@ -1,0 +1,88 @@
+"""Builds the native libraries of gpusgdlm_amd in-tree.
+
+  gpusgdlm_amd/lib/libsgdlm_hip.so   -- HIP kernels + C-ABI (include/sgdlm.h)
+  gpusgdlm_amd/lib/libsgdlm_host.so  -- C++ SGDLM::HostWrapper<double> over it
+
+Usage: python -m gpusgdlm_amd.build [--jobs N]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+OBJ = os.path.join(LIB, "obj")
+
+ARCH = "gfx950"
+HIP_SOURCES = ["k_filter.hip", "k_draws.hip", "k_lu.hip", "k_vb.hip", "sgdlm_capi.hip"]
+HOST_SOURCES = ["host_wrapper.cpp"]
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    # bit-reproducible draws: no implicit FMA contraction anywhere
+    "-ffp-contract=off",
+    # no packed FP32 instructions (v_pk_*_f32) in any kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _hipcc() -> str:
+    p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(p):
+        raise RuntimeError("hipcc not found")
+    return p
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip():
+        sys.stderr.write(out + "\n")
+
+
+def build(jobs: int = 4, verbose: bool = True) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        o = os.path.join(OBJ, src.replace(".hip", ".o"))
+        objs.append(o)
+        cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, cmds))
+    so = os.path.join(LIB, "libsgdlm_hip.so")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", so + ".tmp", *objs])
+    os.replace(so + ".tmp", so)
+    host = os.path.join(LIB, "libsgdlm_host.so")
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    if srcs:
+        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+              *srcs, "-o", host + ".tmp", "-L", LIB, "-lsgdlm_hip", "-Wl,-rpath,$ORIGIN"])
+        os.replace(host + ".tmp", host)
+    if verbose:
+        print(f"built {so}")
+    return so
+
+
+def build_oracle() -> str:
+    d = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", d])
+    return os.path.join(d, "_build", "libsgdlm_oracle.so")
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    a = ap.parse_args()
+    build(a.jobs)

@@ -1,0 +1,206 @@
+/*
+ * k_draws.hip -- naive-posterior draws, forecast right-hand sides, the
+ * multi-step evolution of forecast samples, and the packed <-> reference
+ * theta layout conversion.
+ *
+ * Replaces sampleGamma2/make_gamma2 + curandGenerateNormal + cholesky +
+ * gemmBatched(U^T z) + makeVB_MVN (rSGDLM/src/SGDLM/SGDLM.cu:623-677,
+ * curand_manager.cuh:111-141, kernel_functions.cuh:71-83,568-634,844-902),
+ * batchedComputeNuPlusMu (kernel_functions.cuh:1155-1178) and
+ * evolve_lambdas_and_thetas (SGDLM.cu:679-762).
+ *
+ * Thread mapping: blockIdx.y = series j, consecutive threads = consecutive
+ * draws of that series, so a wavefront shares p_j, n_j, s_j and reads the
+ * same Cholesky factor (broadcast loads) with no loop divergence except the
+ * gamma sampler's rare second attempt.  Per-thread accumulators live in LDS
+ * with the thread index as the fastest dimension (bank-conflict free).
+ * Random numbers come from counter-based Philox, so every (draw, series)
+ * value is independent of launch shape and identical to the host oracle.
+ */
+#include <hip/hip_runtime.h>
+
+#include "sgdlm_internal.h"
+#include "sgdlm_numerics.h"
+
+namespace sgdlm {
+namespace {
+
+constexpr int TB = 128;
+
+__global__ __launch_bounds__(TB) void k_cholesky(int m, int max_p,
+                                                 const unsigned *p,
+                                                 const double *C, double *U) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) {
+    const size_t o = (size_t)j * max_p * max_p;
+    sg_cholesky_upper((int)p[j], max_p, max_p, C + o, U + o);
+  }
+}
+
+__global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
+                                             const unsigned *p, const int *off,
+                                             long P, const double *m_t,
+                                             const double *U, const double *n_t,
+                                             const double *s_t, uint64_t seed,
+                                             uint32_t epoch, double *lam,
+                                             double *theta) {
+  extern __shared__ double sacc[]; /* [max_p][TB] */
+  const int j = blockIdx.y;
+  const long i = (long)blockIdx.x * TB + threadIdx.x;
+  if (i >= N)
+    return;
+  const int pj = (int)p[j];
+  double *acc = sacc + threadIdx.x;
+  const double l = sg_draw_lambda_theta(
+      pj, max_p, m_t + (size_t)j * max_p, U + (size_t)j * max_p * max_p, n_t[j],
+      s_t[j], seed, epoch, (uint32_t)i, (uint32_t)j, acc, TB);
+  lam[(size_t)i * m + j] = l;
+  double *out = theta + (size_t)i * P + off[j];
+  for (int k = 0; k < pj; ++k)
+    out[k] = acc[k * TB];
+}
+
+/* rhs_ij = nu_ij / sqrt(lambda_ij) + sum_{k: sp_jk external} x_jk theta_ijk
+ * (kernel_functions.cuh:1155-1178; theta_k = m_k beyond p_j). */
+__global__ __launch_bounds__(TB) void k_forecast_rhs(
+    long N, int m, int max_p, const unsigned *p, const unsigned *sp,
+    const int *off, long P, const double *x, const double *m_t,
+    const double *theta, const double *lam, uint64_t seed, uint32_t epoch,
+    double *rhs) {
+  const int j = blockIdx.y;
+  const long i = (long)blockIdx.x * TB + threadIdx.x;
+  if (i >= N)
+    return;
+  const int pj = (int)p[j];
+  const unsigned mm = (unsigned)m * (unsigned)m;
+  const double *th = theta + (size_t)i * P + off[j];
+  double nu = sg_normal_k(seed, epoch, SG_PURPOSE_NU, (uint32_t)i, (uint32_t)j, 0u);
+  nu = nu / __builtin_sqrt(lam[(size_t)i * m + j]);
+  for (int k = 0; k < max_p; ++k) {
+    if (sp[(size_t)j * max_p + k] >= mm) {
+      const double t = (k < pj) ? th[k] : m_t[(size_t)j * max_p + k];
+      nu = nu + x[(size_t)j * max_p + k] * t;
+    }
+  }
+  rhs[(size_t)i * m + j] = nu;
+}
+
+/* lambda *= eta (beta draw), omega_k = z_k / sqrt(c lambda) for k < p_j,
+ * theta = G theta (optional), theta += chol(W)^T omega
+ * (SGDLM.cu:679-762 with the intended scale_omegas indexing). */
+__global__ __launch_bounds__(TB) void k_evolve(
+    long N, int m, int max_p, const unsigned *p, const int *off, long P,
+    const double *U_W, const double *r_t, const double *c_t,
+    const double *beta, const double *G, const double *m_t, uint64_t seed,
+    uint32_t epoch, double *lam, double *theta) {
+  extern __shared__ double sbuf[]; /* omega [max_p][TB], old theta [max_p][TB] */
+  const int j = blockIdx.y;
+  const long i = (long)blockIdx.x * TB + threadIdx.x;
+  if (i >= N)
+    return;
+  const int pj = (int)p[j];
+  double *om = sbuf + threadIdx.x;
+  double *ot = sbuf + (size_t)max_p * TB + threadIdx.x;
+  double *th = theta + (size_t)i * P + off[j];
+  const double *U = U_W + (size_t)j * max_p * max_p;
+  double *lp = lam + (size_t)i * m + j;
+  const double eta = sg_beta_eta(r_t[j], beta[j], seed, epoch, (uint32_t)i, (uint32_t)j);
+  const double l = *lp * eta;
+  *lp = l;
+  const double sc = __builtin_sqrt(c_t[j] * l);
+  for (int q = 0; q < pj; q += 2) {
+    double z0, z1;
+    sg_normal_pair(sg_rng_block(seed, epoch, SG_PURPOSE_OMEGA, (uint32_t)i,
+                                (uint32_t)j, (uint32_t)(q >> 1)),
+                   &z0, &z1);
+    om[q * TB] = z0 / sc;
+    if (q + 1 < pj)
+      om[(q + 1) * TB] = z1 / sc;
+  }
+  if (G != nullptr) {
+    const double *Gj = G + (size_t)j * max_p * max_p;
+    const double *mj = m_t + (size_t)j * max_p;
+    for (int k = 0; k < pj; ++k)
+      ot[k * TB] = th[k];
+    for (int k = 0; k < pj; ++k) {
+      double acc = 0.0;
+      for (int n = 0; n < max_p; ++n) {
+        const double v = (n < pj) ? ot[n * TB] : mj[n];
+        acc += Gj[k + n * max_p] * v;
+      }
+      th[k] = acc;
+    }
+  }
+  for (int k = 0; k < pj; ++k) {
+    double acc = 0.0;
+    for (int n = 0; n <= k; ++n)
+      acc = acc + U[k * max_p + n] * om[n * TB];
+    th[k] = acc + th[k];
+  }
+}
+
+__global__ __launch_bounds__(TB) void k_unpack_theta(long N, int m, int max_p,
+                                                     const unsigned *p,
+                                                     const int *off, long P,
+                                                     const double *theta,
+                                                     const double *m_t,
+                                                     double *out) {
+  const int j = blockIdx.y;
+  const long i = (long)blockIdx.x * TB + threadIdx.x;
+  if (i >= N)
+    return;
+  const int pj = (int)p[j];
+  const double *th = theta + (size_t)i * P + off[j];
+  double *o = out + ((size_t)i * m + j) * max_p;
+  for (int k = 0; k < max_p; ++k)
+    o[k] = (k < pj) ? th[k] : m_t[(size_t)j * max_p + k];
+}
+
+dim3 grid_nm(long N, int m) { return dim3((unsigned)((N + TB - 1) / TB), (unsigned)m); }
+
+} // namespace
+
+void launch_cholesky(const Structure &S, const double *C, double *U,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_cholesky, dim3((S.m + TB - 1) / TB), dim3(TB), 0, st,
+                     S.m, S.max_p, S.p, C, U);
+}
+
+void launch_draw(const Structure &S, long N, const double *m_t,
+                 const double *U, const double *n_t, const double *s_t,
+                 uint64_t seed, uint32_t epoch, double *lam, double *theta,
+                 hipStream_t st) {
+  const size_t lds = (size_t)S.max_p * TB * sizeof(double);
+  hipLaunchKernelGGL(k_draw, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
+                     S.max_p, S.p, S.off, S.P, m_t, U, n_t, s_t, seed, epoch,
+                     lam, theta);
+}
+
+void launch_forecast_rhs(const Structure &S, long N, const double *x,
+                         const double *m_t, const double *theta,
+                         const double *lam, uint64_t seed, uint32_t epoch,
+                         double *rhs, hipStream_t st) {
+  hipLaunchKernelGGL(k_forecast_rhs, grid_nm(N, S.m), dim3(TB), 0, st, N, S.m,
+                     S.max_p, S.p, S.sp, S.off, S.P, x, m_t, theta, lam, seed,
+                     epoch, rhs);
+}
+
+void launch_evolve(const Structure &S, long N, const double *U_W,
+                   const double *r_t, const double *c_t, const double *beta,
+                   const double *G, const double *m_t, uint64_t seed,
+                   uint32_t epoch, double *lam, double *theta,
+                   hipStream_t st) {
+  const size_t lds = 2 * (size_t)S.max_p * TB * sizeof(double);
+  hipLaunchKernelGGL(k_evolve, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
+                     S.max_p, S.p, S.off, S.P, U_W, r_t, c_t, beta, G, m_t,
+                     seed, epoch, lam, theta);
+}
+
+void launch_unpack_theta(const Structure &S, long N, const double *theta,
+                         const double *m_t, double *theta_full,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_unpack_theta, grid_nm(N, S.m), dim3(TB), 0, st, N, S.m,
+                     S.max_p, S.p, S.off, S.P, theta, m_t, theta_full);
+}
+
+} // namespace sgdlm

@@ -1,0 +1,459 @@
+/*
+ * k_vb.hip -- importance weights and the VB moment-match.
+ *
+ * Replaces, from rSGDLM/src/SGDLM/SGDLM.cu:325-503:
+ *   sumVector<<<1,1>>> + scale (single-thread normalisation, Q2), by an FP64
+ *     log-sum-exp over the log-determinants: w_i = exp(l_i - max) / sum,
+ *     non-finite l_i -> weight 0, ESS = 1 / sum w^2 (one workgroup, fixed
+ *     reduction tree, deterministic);
+ *   meanScalarAndLog / compute_mean_m_t / compute_VB_vector1 /
+ *     meanBatchedVecVecT (serial-over-N per-thread loops,
+ *     kernel_functions.cuh:928-982,1102-1133), by two chunked passes over
+ *     the packed draws: pass 1 sums w*lambda, w*log(lambda), w*lambda*theta;
+ *     pass 2 forms V = sum w lambda (theta - m)(theta - m)' as a
+ *     p x N . N x p contraction on FP64 MFMA (v_mfma_f64_16x16x4f64, the
+ *     4 draws of a k-step on the contraction axis);
+ *   getrf/getri(max_p) + the two m*N gemmBatched calls for Q and
+ *     compute_mean_Q_t (SGDLM.cu:404-465), by a Gauss-Jordan inverse of the
+ *     p x p block and Qbar = trace(V^{-1} V), which equals
+ *     sum_i w_i lambda_i (theta_i - m)' V^{-1} (theta_i - m) identically;
+ *   compute_mean_n_t / compute_mean_s_t / batchedScale (Newton with the
+ *     truncated psi/psi1 series, OPTIM_PRECISION 1e-6, OPTIM_MAX_ITER 50,
+ *     fallback to n_t; s < 0 -> s_t; C = s V on the block;
+ *     kernel_functions.cuh:1006-1085).
+ * Partial sums are combined in a fixed chunk order, so results are
+ * bit-reproducible run to run.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "sgdlm_internal.h"
+
+#define OPTIM_PRECISION 1e-6 /* rSGDLM/src/Makevars.in:1 */
+#define OPTIM_MAX_ITER 50    /* rSGDLM/src/Makevars.in:1 */
+
+namespace sgdlm {
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int WTB = 1024;
+
+__device__ inline bool finite_d(double v) { return v == v && fabs(v) <= 1.79769313486231570815e308; }
+
+__device__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v += __shfl_xor(v, o);
+  return v;
+}
+__device__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+/* one workgroup: log-sum-exp weights */
+__global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
+                                                 double *w, double *stats) {
+  __shared__ double red[WTB / 64];
+  __shared__ double bc;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double mx = -INFINITY;
+  for (long i = tid; i < N; i += WTB) {
+    const double v = ld[i];
+    if (finite_d(v) && v > mx)
+      mx = v;
+  }
+  mx = wave_max(mx);
+  if (lane == 0)
+    red[wv] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    double t = red[0];
+    for (int q = 1; q < WTB / 64; ++q)
+      t = fmax(t, red[q]);
+    bc = t;
+  }
+  __syncthreads();
+  mx = bc;
+  double s = 0.0;
+  for (long i = tid; i < N; i += WTB) {
+    const double v = ld[i];
+    if (finite_d(v))
+      s += exp(v - mx);
+  }
+  s = wave_sum(s);
+  __syncthreads();
+  if (lane == 0)
+    red[wv] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < WTB / 64; ++q)
+      t += red[q];
+    bc = t;
+  }
+  __syncthreads();
+  s = bc;
+  double s2 = 0.0, nz = 0.0;
+  for (long i = tid; i < N; i += WTB) {
+    const double v = ld[i];
+    const double wi = finite_d(v) ? exp(v - mx) / s : (s > 0.0 ? 0.0 : NAN);
+    w[i] = wi;
+    s2 += wi * wi;
+    nz += (wi == 0.0) ? 1.0 : 0.0;
+  }
+  s2 = wave_sum(s2);
+  nz = wave_sum(nz);
+  __syncthreads();
+  if (lane == 0)
+    red[wv] = s2;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < WTB / 64; ++q)
+      t += red[q];
+    stats[2] = t;
+    stats[0] = mx;
+    stats[1] = s;
+  }
+  __syncthreads();
+  if (lane == 0)
+    red[wv] = nz;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < WTB / 64; ++q)
+      t += red[q];
+    stats[3] = t;
+  }
+}
+
+/* pass 1: per (chunk, series) sums of w*lambda*theta_k, w*lambda,
+ * w*log(lambda).  threads = 4 draw groups x 64 coefficient lanes. */
+__global__ __launch_bounds__(256) void k_mom1(long N, int m, int max_p,
+                                              int chunks, const unsigned *p,
+                                              const int *off, long P,
+                                              const double *w,
+                                              const double *lam,
+                                              const double *theta,
+                                              double *part1) {
+  __shared__ double red[4][66];
+  const int j = blockIdx.y, c = blockIdx.x;
+  const long i0 = N * c / chunks, i1 = N * (c + 1) / chunks;
+  const int tid = threadIdx.x, kl = tid & 63, dg = tid >> 6;
+  const int pj = (int)p[j];
+  const double *th = theta + off[j];
+  double at = 0.0, al = 0.0, ll = 0.0;
+  for (long i = i0 + dg; i < i1; i += 4) {
+    const double wi = w[i];
+    const double li = lam[(size_t)i * m + j];
+    const double wl = wi * li;
+    if (kl < pj)
+      at += wl * th[(size_t)i * P + kl];
+    if (kl == 0) {
+      al += wl;
+      ll += wi * log(li);
+    }
+  }
+  red[dg][kl] = at;
+  if (kl == 0) {
+    red[dg][64] = al;
+    red[dg][65] = ll;
+  }
+  __syncthreads();
+  double *out = part1 + ((size_t)c * m + j) * (max_p + 2);
+  if (dg == 0) {
+    if (kl < max_p)
+      out[kl] = ((red[0][kl] + red[1][kl]) + red[2][kl]) + red[3][kl];
+    if (kl == 0) {
+      out[max_p] = ((red[0][64] + red[1][64]) + red[2][64]) + red[3][64];
+      out[max_p + 1] = ((red[0][65] + red[1][65]) + red[2][65]) + red[3][65];
+    }
+  }
+}
+
+/* combine pass-1 partials of series j in chunk order */
+__device__ void series_means(int j, int m, int max_p, int chunks, int pj,
+                             const double *part1, int k, double *lbar,
+                             double *llbar, double *mk) {
+  double sl = 0.0, sll = 0.0, st = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    const double *pp = part1 + ((size_t)c * m + j) * (max_p + 2);
+    sl += pp[max_p];
+    sll += pp[max_p + 1];
+    if (k < pj)
+      st += pp[k];
+  }
+  *lbar = sl;
+  *llbar = sll;
+  *mk = (k < pj) ? st / sl : 0.0;
+}
+
+/* pass 2: V partials on FP64 MFMA.  NBLK = number of 16-wide coefficient
+ * blocks (p_j <= 16 NBLK). */
+template <int NBLK>
+__global__ __launch_bounds__(256) void k_mom2(long N, int m, int max_p,
+                                              int chunks, const unsigned *p,
+                                              const int *off, long P,
+                                              const double *w,
+                                              const double *lam,
+                                              const double *theta,
+                                              const double *part1,
+                                              double *part2) {
+  constexpr int NT = NBLK * (NBLK + 1) / 2;
+  __shared__ double mb[16 * NBLK];
+  __shared__ double tiles[NT][256];
+  const int j = blockIdx.y, c = blockIdx.x;
+  const long i0 = N * c / chunks, i1 = N * (c + 1) / chunks;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int pj = (int)p[j];
+  if (tid < 16 * NBLK) {
+    double lb, llb, mk;
+    series_means(j, m, max_p, chunks, pj, part1, tid, &lb, &llb, &mk);
+    mb[tid] = mk;
+  }
+  __syncthreads();
+  f64x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+    acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const double *th = theta + off[j];
+  const long nsteps = (i1 - i0 + 3) / 4;
+  for (long s = wave; s < nsteps; s += 4) {
+    const long i = i0 + 4 * s + g;
+    double x[NBLK];
+    double sc = 0.0;
+    if (i < i1)
+      sc = sqrt(w[i] * lam[(size_t)i * m + j]);
+#pragma unroll
+    for (int b = 0; b < NBLK; ++b) {
+      const int k = 16 * b + l16;
+      x[b] = (i < i1 && k < pj) ? sc * (th[(size_t)i * P + k] - mb[k]) : 0.0;
+    }
+    int t = 0;
+#pragma unroll
+    for (int A = 0; A < NBLK; ++A)
+#pragma unroll
+      for (int B = A; B < NBLK; ++B) {
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[A], x[B], acc[t], 0, 0, 0);
+        ++t;
+      }
+  }
+  /* sum the 4 waves' tiles in a fixed order */
+  for (int wv = 0; wv < 4; ++wv) {
+    if (wave == wv) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = (4 * g + r) * 16 + l16;
+          tiles[t][e] = (wv == 0) ? acc[t][r] : tiles[t][e] + acc[t][r];
+        }
+    }
+    __syncthreads();
+  }
+  double *out = part2 + ((size_t)c * m + j) * max_p * max_p;
+  int t = 0;
+  for (int A = 0; A < NBLK; ++A)
+    for (int B = A; B < NBLK; ++B) {
+      for (int e = tid; e < 256; e += 256) {
+        const int r = 16 * A + e / 16, col = 16 * B + (e & 15);
+        if (r < pj && col < pj) {
+          out[r + (size_t)col * max_p] = tiles[t][e];
+          out[col + (size_t)r * max_p] = tiles[t][e];
+        }
+      }
+      ++t;
+    }
+}
+
+__device__ double psi_series(double x) {
+  return log(x) - 1.0 / (2.0 * x) - 1.0 / (12.0 * x * x) +
+         1.0 / (120.0 * x * x * x * x) - 1.0 / (252.0 * x * x * x * x * x * x) +
+         1.0 / (240.0 * x * x * x * x * x * x * x * x) -
+         5.0 / (660.0 * x * x * x * x * x * x * x * x * x * x) +
+         691.0 / (32760.0 * x * x * x * x * x * x * x * x * x * x * x * x) -
+         1.0 / (12.0 * x * x * x * x * x * x * x * x * x * x * x * x * x * x);
+}
+__device__ double psi1_series(double x) {
+  return 1.0 / x + 1.0 / (2.0 * x * x) + 1.0 / (6.0 * x * x * x) -
+         1.0 / (30.0 * x * x * x * x * x) +
+         1.0 / (42.0 * x * x * x * x * x * x * x) -
+         1.0 / (30.0 * x * x * x * x * x * x * x * x * x);
+}
+
+/* final per-series moment-match; one wavefront per series */
+__global__ __launch_bounds__(64) void k_vb_final(
+    int m, int max_p, int chunks, const unsigned *p, const double *n_t,
+    const double *s_t, const double *part1, const double *part2,
+    double *mean_lambda, double *mean_log_lambda, double *mean_m,
+    double *mean_C, double *mean_n, double *mean_s, double *mean_Q,
+    int *newton_iter) {
+  extern __shared__ double sh[];
+  const int j = blockIdx.x, t = threadIdx.x;
+  const int pj = (int)p[j], mp = max_p, mp2 = max_p * max_p;
+  double *V = sh;                     /* pj x pj (ld pj) */
+  double *Aug = V + (size_t)pj * pj;  /* pj x 2pj        */
+  __shared__ double sc[4];
+  __shared__ int spiv;
+  double lb = 0.0, llb = 0.0, mk = 0.0;
+  for (int k = t; k < mp; k += 64) {
+    series_means(j, m, max_p, chunks, pj, part1, k, &lb, &llb, &mk);
+    mean_m[(size_t)j * mp + k] = mk;
+  }
+  if (t == 0) {
+    series_means(j, m, max_p, chunks, pj, part1, 0, &lb, &llb, &mk);
+    sc[0] = lb;
+    sc[1] = llb;
+  }
+  for (int e = t; e < pj * pj; e += 64) {
+    const int r = e % pj, c = e / pj;
+    double v = 0.0;
+    for (int ch = 0; ch < chunks; ++ch)
+      v += part2[((size_t)ch * m + j) * mp2 + r + (size_t)c * mp];
+    V[e] = v;
+  }
+  __syncthreads();
+  for (int e = t; e < pj * 2 * pj; e += 64) {
+    const int r = e / (2 * pj), c = e % (2 * pj);
+    Aug[e] = (c < pj) ? V[r + c * pj] : ((c - pj == r) ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  /* Gauss-Jordan with partial pivoting */
+  for (int k = 0; k < pj; ++k) {
+    if (t == 0) {
+      int piv = k;
+      double best = fabs(Aug[(size_t)k * 2 * pj + k]);
+      for (int r = k + 1; r < pj; ++r) {
+        const double v = fabs(Aug[(size_t)r * 2 * pj + k]);
+        if (v > best) {
+          best = v;
+          piv = r;
+        }
+      }
+      spiv = piv;
+    }
+    __syncthreads();
+    const int piv = spiv;
+    if (piv != k)
+      for (int c = t; c < 2 * pj; c += 64) {
+        const double tmp = Aug[(size_t)k * 2 * pj + c];
+        Aug[(size_t)k * 2 * pj + c] = Aug[(size_t)piv * 2 * pj + c];
+        Aug[(size_t)piv * 2 * pj + c] = tmp;
+      }
+    __syncthreads();
+    const double d = Aug[(size_t)k * 2 * pj + k];
+    __syncthreads();
+    for (int c = t; c < 2 * pj; c += 64)
+      Aug[(size_t)k * 2 * pj + c] /= d;
+    __syncthreads();
+    for (int r = t; r < pj; r += 64) {
+      if (r != k) {
+        const double f = Aug[(size_t)r * 2 * pj + k];
+        for (int c = 0; c < 2 * pj; ++c)
+          Aug[(size_t)r * 2 * pj + c] -= f * Aug[(size_t)k * 2 * pj + c];
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    /* Qbar = trace(V^{-1} V) */
+    double Q = 0.0;
+    for (int a = 0; a < pj; ++a)
+      for (int b = 0; b < pj; ++b)
+        Q += Aug[(size_t)a * 2 * pj + pj + b] * V[b + a * pj];
+    const double lbar = sc[0], llbar = sc[1];
+    const double pd = (double)pj;
+    int iter = 0;
+    double mn = n_t[j], fun, dfun, step = 0.0;
+    do {
+      mn -= step;
+      fun = log(mn + pd - Q) - psi_series(mn / 2.0) - (pd - Q) / mn -
+            log(2.0 * lbar) + llbar;
+      dfun = 1.0 / (mn + pd - Q) - 0.5 * psi1_series(mn / 2.0) +
+             (pd - Q) / (mn * mn);
+      step = fun / dfun;
+      if (step > mn)
+        step = 0.5 * mn;
+    } while (fabs(fun) > OPTIM_PRECISION && iter++ < OPTIM_MAX_ITER);
+    const double nbar = (iter >= OPTIM_MAX_ITER) ? n_t[j] : mn;
+    double sbar = (nbar + (double)pj - Q) / (nbar * lbar);
+    if (sbar < 0.0)
+      sbar = s_t[j];
+    mean_lambda[j] = lbar;
+    mean_log_lambda[j] = llbar;
+    mean_n[j] = nbar;
+    mean_s[j] = sbar;
+    mean_Q[j] = Q;
+    newton_iter[j] = iter;
+    sc[2] = sbar;
+  }
+  __syncthreads();
+  const double sbar = sc[2];
+  for (int e = t; e < mp2; e += 64) {
+    const int r = e % mp, c = e / mp;
+    mean_C[(size_t)j * mp2 + e] = (r < pj && c < pj) ? sbar * V[r + c * pj] : 0.0;
+  }
+}
+
+} // namespace
+
+int vb_chunks(long N) {
+  long c = N / 2048;
+  if (c < 1)
+    c = 1;
+  if (c > 64)
+    c = 64;
+  return (int)c;
+}
+
+void launch_weights(long N, const double *logdet, double *w, double *stats,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_weights, dim3(1), dim3(WTB), 0, st, N, logdet, w, stats);
+}
+
+void launch_moments(const Structure &S, long N, const double *lam,
+                    const double *theta, const double *n_t, const double *s_t,
+                    VbBuffers &B, hipStream_t st) {
+  const int chunks = B.chunks;
+  hipLaunchKernelGGL(k_mom1, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
+                     S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
+                     B.part1);
+  const int nblk = (S.max_p + 15) / 16;
+  switch (nblk) {
+  case 1:
+    hipLaunchKernelGGL(k_mom2<1>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
+                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
+                       B.part1, B.part2);
+    break;
+  case 2:
+    hipLaunchKernelGGL(k_mom2<2>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
+                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
+                       B.part1, B.part2);
+    break;
+  case 3:
+    hipLaunchKernelGGL(k_mom2<3>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
+                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
+                       B.part1, B.part2);
+    break;
+  default:
+    hipLaunchKernelGGL(k_mom2<4>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
+                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
+                       B.part1, B.part2);
+    break;
+  }
+  const size_t lds = 3 * (size_t)S.max_p * S.max_p * sizeof(double);
+  (void)hipFuncSetAttribute((const void *)k_vb_final,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_vb_final, dim3(S.m), dim3(64), lds, st, S.m, S.max_p,
+                     chunks, S.p, n_t, s_t, B.part1, B.part2, B.mean_lambda,
+                     B.mean_log_lambda, B.mean_m, B.mean_C, B.mean_n,
+                     B.mean_s, B.mean_Q, B.newton_iter);
+}
+
+} // namespace sgdlm

@@ -1,0 +1,1027 @@
+/*
+ * sgdlm_capi.hip -- the C-ABI of libsgdlm_hip.so: context, state machine,
+ * device buffers and stage sequencing.
+ *
+ * Plays the role of SGDLM::HostWrapperImpl<double>
+ * (rSGDLM/src/SGDLM/HostWrapperImpl.cu) for one GPU: the same state flags
+ * (memory_initialized, sim_memory_initialized, evo_memory_initialized,
+ * is_prior, use_state_evolution_matrix; HostWrapperImpl.cuh:101-127) and the
+ * same call-order rules (checkInitialized / checkSimInitialized / checkPrior
+ * / checkUseStateEvolutionMatrix, HostWrapperImpl.cu:1239-1286), but errors
+ * are returned as status codes instead of being logged and ignored.
+ *
+ * Device data are plain strided arrays (no pointer arrays); draws are kept
+ * in a packed draw-major layout theta[i * P + off_j + k], P = sum_j p_j.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sgdlm.h"
+#include "sgdlm_internal.h"
+
+using namespace sgdlm;
+
+namespace {
+
+struct DBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct PendingEvent {
+  std::string stage;
+  hipEvent_t a, b;
+};
+
+} // namespace
+
+struct sgdlm_ctx {
+  std::string err;
+  int device = 0;
+  int n_cu = 0;
+  size_t lds_max = 65536;
+  hipStream_t stream = nullptr;
+
+  size_t m = 0, max_p = 0;
+  bool memory_initialized = false;
+  bool sim_memory_initialized = false;
+  bool evo_memory_initialized = false;
+  bool use_state_evolution_matrix = false;
+  bool is_prior = false;
+  bool parents_set = false;
+  size_t nsim = 0, nsim_batch = 0;
+
+  int prec = SGDLM_PREC_FP64;
+  uint64_t seed = 1234ull;
+  uint32_t epoch = 0;
+
+  std::vector<unsigned> h_p, h_sp;
+  Structure S;
+
+  /* model memory */
+  std::map<std::string, DBuf> mem;
+  /* simulation memory */
+  std::map<std::string, DBuf> sim;
+  bool vb_valid = false;
+
+  bool timing = false;
+  std::map<std::string, std::pair<double, long>> times;
+  std::vector<PendingEvent> pending;
+};
+
+namespace {
+
+int fail(sgdlm_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail((ctx), SGDLM_ERR_DEVICE, "%s: %s", #call,                    \
+                  hipGetErrorString(e_));                                      \
+  } while (0)
+
+int dalloc(sgdlm_ctx *c, std::map<std::string, DBuf> &pool,
+           const std::string &name, size_t bytes) {
+  DBuf &b = pool[name];
+  if (b.ptr && b.bytes >= bytes)
+    return SGDLM_OK;
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0)
+    bytes = 16;
+  hipError_t e = hipMalloc(&b.ptr, bytes);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(c, SGDLM_ERR_ALLOC, "device allocation of %zu bytes for %s failed: %s",
+                bytes, name.c_str(), hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return SGDLM_OK;
+}
+
+void free_pool(std::map<std::string, DBuf> &pool) {
+  for (auto &kv : pool)
+    if (kv.second.ptr)
+      (void)hipFree(kv.second.ptr);
+  pool.clear();
+}
+
+template <typename T> T *M_(sgdlm_ctx *c, const char *n) {
+  return c->mem[n].as<T>();
+}
+template <typename T> T *SIM(sgdlm_ctx *c, const char *n) {
+  return c->sim[n].as<T>();
+}
+
+struct StageTimer {
+  sgdlm_ctx *c;
+  std::string stage;
+  hipEvent_t a{}, b{};
+  bool on;
+  StageTimer(sgdlm_ctx *ctx, const char *s) : c(ctx), stage(s), on(ctx->timing) {
+    if (on) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (on) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({stage, a, b});
+    }
+  }
+};
+
+int sync_and_collect(sgdlm_ctx *c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &pe : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+      auto &t = c->times[pe.stage];
+      t.first += ms;
+      t.second += 1;
+    }
+    (void)hipEventDestroy(pe.a);
+    (void)hipEventDestroy(pe.b);
+  }
+  c->pending.clear();
+  return SGDLM_OK;
+}
+
+#define CHECK_LAUNCH(ctx)                                                      \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess)                                                      \
+      return fail((ctx), SGDLM_ERR_DEVICE, "kernel launch: %s",                \
+                  hipGetErrorString(e_));                                      \
+  } while (0)
+
+int need_init(const sgdlm_ctx *c) {
+  return c->memory_initialized ? SGDLM_OK : SGDLM_ERR_STATE;
+}
+
+int check_init(sgdlm_ctx *c) {
+  if (!c->memory_initialized)
+    return fail(c, SGDLM_ERR_STATE, "memory not initialised (call init_memory / set parents first)");
+  return SGDLM_OK;
+}
+int check_parents(sgdlm_ctx *c) {
+  if (!c->parents_set)
+    return fail(c, SGDLM_ERR_STATE, "parental sets not set");
+  return SGDLM_OK;
+}
+int check_sim(sgdlm_ctx *c) {
+  if (!c->sim_memory_initialized)
+    return fail(c, SGDLM_ERR_STATE, "simulation memory not initialised");
+  return SGDLM_OK;
+}
+int check_prior(sgdlm_ctx *c, bool want_prior) {
+  if (want_prior && !c->is_prior)
+    return fail(c, SGDLM_ERR_STATE, "parameters are posterior parameters; a prior is required");
+  if (!want_prior && c->is_prior)
+    return fail(c, SGDLM_ERR_STATE, "parameters are prior parameters; a posterior is required");
+  return SGDLM_OK;
+}
+int check_evo(sgdlm_ctx *c) {
+  if (!c->use_state_evolution_matrix)
+    return fail(c, SGDLM_ERR_STATE, "state evolution matrix not enabled");
+  return SGDLM_OK;
+}
+
+/* Allocate the simulation buffers for the current N and P. */
+int ensure_sim_buffers(sgdlm_ctx *c) {
+  const size_t N = c->nsim, m = c->m, mp = c->max_p;
+  const size_t P = (size_t)c->S.P;
+  int rc;
+  if ((rc = dalloc(c, c->sim, "lam", N * m * 8)) ||
+      (rc = dalloc(c, c->sim, "theta", N * std::max<size_t>(P, 1) * 8)) ||
+      (rc = dalloc(c, c->sim, "logdet", N * 8)) ||
+      (rc = dalloc(c, c->sim, "w", N * 8)) ||
+      (rc = dalloc(c, c->sim, "stats", 8 * 8)))
+    return rc;
+  const int chunks = vb_chunks((long)N);
+  if ((rc = dalloc(c, c->sim, "part1", (size_t)chunks * m * (mp + 2) * 8)) ||
+      (rc = dalloc(c, c->sim, "part2", (size_t)chunks * m * mp * mp * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_lambda", m * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_log_lambda", m * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_m", m * mp * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_C", m * mp * mp * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_n", m * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_s", m * 8)) ||
+      (rc = dalloc(c, c->sim, "mean_Q", m * 8)) ||
+      (rc = dalloc(c, c->sim, "newton_iter", m * 4)))
+    return rc;
+  return SGDLM_OK;
+}
+
+VbBuffers vb_buffers(sgdlm_ctx *c) {
+  VbBuffers B;
+  B.w = SIM<double>(c, "w");
+  B.stats = SIM<double>(c, "stats");
+  B.part1 = SIM<double>(c, "part1");
+  B.part2 = SIM<double>(c, "part2");
+  B.mean_lambda = SIM<double>(c, "mean_lambda");
+  B.mean_log_lambda = SIM<double>(c, "mean_log_lambda");
+  B.mean_m = SIM<double>(c, "mean_m");
+  B.mean_C = SIM<double>(c, "mean_C");
+  B.mean_n = SIM<double>(c, "mean_n");
+  B.mean_s = SIM<double>(c, "mean_s");
+  B.mean_Q = SIM<double>(c, "mean_Q");
+  B.newton_iter = SIM<int>(c, "newton_iter");
+  B.chunks = vb_chunks((long)c->nsim);
+  return B;
+}
+
+int h2d(sgdlm_ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return SGDLM_OK;
+}
+int d2h(sgdlm_ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return SGDLM_OK;
+}
+
+/* Rebuild the device-side parent structure from h_p / h_sp. */
+int build_structure(sgdlm_ctx *c) {
+  const int m = (int)c->m, mp = (int)c->max_p;
+  const unsigned mm = (unsigned)m * (unsigned)m;
+  std::vector<int> off(m + 1, 0);
+  for (int j = 0; j < m; ++j)
+    off[j + 1] = off[j] + (int)c->h_p[j];
+  struct E {
+    int row, col, idx;
+  };
+  std::vector<E> ents;
+  for (int j = 0; j < m; ++j)
+    for (int k = 0; k < (int)c->h_p[j]; ++k) {
+      const unsigned v = c->h_sp[(size_t)j * mp + k];
+      if (v < mm)
+        ents.push_back({(int)(v % (unsigned)m), (int)(v / (unsigned)m), off[j] + k});
+    }
+  std::sort(ents.begin(), ents.end(), [](const E &a, const E &b) {
+    return a.row != b.row ? a.row < b.row : a.col < b.col;
+  });
+  const int ng = (int)ents.size();
+  std::vector<int> gr(std::max(ng, 1)), gc(std::max(ng, 1)), gi(std::max(ng, 1)),
+      rp(m + 1, 0);
+  for (int e = 0; e < ng; ++e) {
+    gr[e] = ents[e].row;
+    gc[e] = ents[e].col;
+    gi[e] = ents[e].idx;
+    rp[ents[e].row + 1]++;
+  }
+  for (int r = 0; r < m; ++r)
+    rp[r + 1] += rp[r];
+  int rc;
+  if ((rc = dalloc(c, c->mem, "off", (m + 1) * 4)) ||
+      (rc = dalloc(c, c->mem, "g_row", std::max(ng, 1) * 4)) ||
+      (rc = dalloc(c, c->mem, "g_col", std::max(ng, 1) * 4)) ||
+      (rc = dalloc(c, c->mem, "g_idx", std::max(ng, 1) * 4)) ||
+      (rc = dalloc(c, c->mem, "g_rowptr", (m + 1) * 4)))
+    return rc;
+  if ((rc = h2d(c, M_<int>(c, "off"), off.data(), (m + 1) * 4)) ||
+      (rc = h2d(c, M_<int>(c, "g_row"), gr.data(), std::max(ng, 1) * 4)) ||
+      (rc = h2d(c, M_<int>(c, "g_col"), gc.data(), std::max(ng, 1) * 4)) ||
+      (rc = h2d(c, M_<int>(c, "g_idx"), gi.data(), std::max(ng, 1) * 4)) ||
+      (rc = h2d(c, M_<int>(c, "g_rowptr"), rp.data(), (m + 1) * 4)))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Structure &S = c->S;
+  S.m = m;
+  S.max_p = mp;
+  S.M = (m + 15) / 16 * 16;
+  S.P = off[m];
+  S.n_gam = ng;
+  S.p = M_<unsigned>(c, "p");
+  S.sp = M_<unsigned>(c, "sp");
+  S.off = M_<int>(c, "off");
+  S.g_row = M_<int>(c, "g_row");
+  S.g_col = M_<int>(c, "g_col");
+  S.g_idx = M_<int>(c, "g_idx");
+  S.g_rowptr = M_<int>(c, "g_rowptr");
+  return SGDLM_OK;
+}
+
+int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
+  int rc = lu_plan(c->S, prec, solve, c->n_cu, c->lds_max, plan);
+  if (rc)
+    return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
+  return dalloc(c, c->sim, "lu_work", plan->work_bytes);
+}
+
+/* The VB stages on the device (no host copies). */
+int run_vb(sgdlm_ctx *c) {
+  int rc;
+  if ((rc = ensure_sim_buffers(c)))
+    return rc;
+  const long N = (long)c->nsim;
+  const uint32_t ep = ++c->epoch;
+  {
+    StageTimer t(c, "draw");
+    launch_cholesky(c->S, M_<double>(c, "C"), M_<double>(c, "U"), c->stream);
+    launch_draw(c->S, N, M_<double>(c, "m"), M_<double>(c, "U"),
+                M_<double>(c, "n"), M_<double>(c, "s"), c->seed, ep,
+                SIM<double>(c, "lam"), SIM<double>(c, "theta"), c->stream);
+  }
+  CHECK_LAUNCH(c);
+  LuPlan plan;
+  if ((rc = lu_workspace(c, c->prec, 0, &plan)))
+    return rc;
+  {
+    StageTimer t(c, "lu");
+    rc = launch_lu_logdet(c->S, plan, N, SIM<double>(c, "theta"),
+                          c->sim["lu_work"].ptr, SIM<double>(c, "logdet"),
+                          c->stream);
+  }
+  if (rc)
+    return fail(c, rc, "LU launch failed: %s", hipGetErrorString(hipGetLastError()));
+  VbBuffers B = vb_buffers(c);
+  {
+    StageTimer t(c, "weights");
+    launch_weights(N, SIM<double>(c, "logdet"), B.w, B.stats, c->stream);
+  }
+  CHECK_LAUNCH(c);
+  {
+    StageTimer t(c, "moments");
+    launch_moments(c->S, N, SIM<double>(c, "lam"), SIM<double>(c, "theta"),
+                   M_<double>(c, "n"), M_<double>(c, "s"), B, c->stream);
+  }
+  CHECK_LAUNCH(c);
+  c->vb_valid = true;
+  return SGDLM_OK;
+}
+
+} // namespace
+
+/* ======================================================================== */
+extern "C" {
+
+int sgdlm_create(sgdlm_ctx **out, size_t no_gpus) {
+  if (!out)
+    return SGDLM_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    return SGDLM_ERR_DEVICE;
+  }
+  (void)no_gpus; /* one GPU per context */
+  sgdlm_ctx *c = new sgdlm_ctx();
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return SGDLM_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+    delete c;
+    return SGDLM_ERR_DEVICE;
+  }
+  c->n_cu = prop.multiProcessorCount;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess &&
+        v > 0)
+      c->lds_max = (size_t)v;
+    if (prop.maxSharedMemoryPerMultiProcessor > c->lds_max)
+      c->lds_max = prop.maxSharedMemoryPerMultiProcessor;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return SGDLM_ERR_DEVICE;
+  }
+  *out = c;
+  return SGDLM_OK;
+}
+
+int sgdlm_destroy(sgdlm_ctx *c) {
+  if (!c)
+    return SGDLM_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_pool(c->sim);
+  free_pool(c->mem);
+  for (auto &pe : c->pending) {
+    (void)hipEventDestroy(pe.a);
+    (void)hipEventDestroy(pe.b);
+  }
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SGDLM_OK;
+}
+
+const char *sgdlm_last_error(const sgdlm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int sgdlm_get_no_series(const sgdlm_ctx *c, size_t *m) {
+  if (!c || !m)
+    return SGDLM_ERR_ARG;
+  *m = c->m;
+  return SGDLM_OK;
+}
+int sgdlm_get_max_p(const sgdlm_ctx *c, size_t *mp) {
+  if (!c || !mp)
+    return SGDLM_ERR_ARG;
+  *mp = c->max_p;
+  return SGDLM_OK;
+}
+int sgdlm_get_evolution_matrix_configuration(const sgdlm_ctx *c, int *u) {
+  if (!c || !u)
+    return SGDLM_ERR_ARG;
+  *u = c->use_state_evolution_matrix ? 1 : 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_clear_memory(sgdlm_ctx *c) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  free_pool(c->sim);
+  free_pool(c->mem);
+  c->memory_initialized = c->sim_memory_initialized = c->evo_memory_initialized = false;
+  c->parents_set = false;
+  c->vb_valid = false;
+  c->nsim = c->nsim_batch = 0;
+  c->S = Structure();
+  return SGDLM_OK;
+}
+
+int sgdlm_init_memory(sgdlm_ctx *c, size_t m, size_t max_p) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (m < 1 || max_p < 1)
+    return fail(c, SGDLM_ERR_ARG, "m and max_p must be positive");
+  if (max_p > 64)
+    return fail(c, SGDLM_ERR_ARG, "max_p=%zu exceeds the supported 64", max_p);
+  if ((unsigned long long)m * m >= 0x80000000ull)
+    return fail(c, SGDLM_ERR_ARG, "m=%zu too large (m*m must stay below 2^31)", m);
+  sgdlm_clear_memory(c);
+  c->m = m;
+  c->max_p = max_p;
+  const size_t mp2 = max_p * max_p;
+  int rc;
+  const char *mats[] = {"delta", "C", "W", "fC", "fW", "U"};
+  const char *vecs_mp[] = {"m", "fm", "F", "x"};
+  const char *vecs_m[] = {"beta", "n", "s", "fn", "fs", "y"};
+  for (const char *n : mats)
+    if ((rc = dalloc(c, c->mem, n, m * mp2 * 8)))
+      return rc;
+  for (const char *n : vecs_mp)
+    if ((rc = dalloc(c, c->mem, n, m * max_p * 8)))
+      return rc;
+  for (const char *n : vecs_m)
+    if ((rc = dalloc(c, c->mem, n, m * 8)))
+      return rc;
+  if ((rc = dalloc(c, c->mem, "p", m * 4)) ||
+      (rc = dalloc(c, c->mem, "sp", m * max_p * 4)))
+    return rc;
+  /* zero everything so getters before setters return zeros */
+  for (auto &kv : c->mem)
+    HIPCHK(c, hipMemsetAsync(kv.second.ptr, 0, kv.second.bytes, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->h_p.assign(m, 0);
+  c->h_sp.assign(m * max_p, 0x80000000u);
+  c->memory_initialized = true;
+  if (c->use_state_evolution_matrix) {
+    c->use_state_evolution_matrix = false;
+    return sgdlm_manage_evo_memory(c, 1);
+  }
+  return SGDLM_OK;
+}
+
+int sgdlm_manage_evo_memory(sgdlm_ctx *c, int use) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (use) {
+    if (c->memory_initialized && !c->evo_memory_initialized) {
+      int rc = dalloc(c, c->mem, "G", c->m * c->max_p * c->max_p * 8);
+      if (rc)
+        return rc;
+      HIPCHK(c, hipMemset(c->mem["G"].ptr, 0, c->mem["G"].bytes));
+    }
+  } else {
+    auto it = c->mem.find("G");
+    if (it != c->mem.end()) {
+      (void)hipFree(it->second.ptr);
+      c->mem.erase(it);
+    }
+  }
+  c->use_state_evolution_matrix = use != 0;
+  c->evo_memory_initialized = use != 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_get_nsim(const sgdlm_ctx *c, size_t *n) {
+  if (!c || !n)
+    return SGDLM_ERR_ARG;
+  *n = c->nsim;
+  return SGDLM_OK;
+}
+int sgdlm_get_nsim_batch(const sgdlm_ctx *c, size_t *n) {
+  if (!c || !n)
+    return SGDLM_ERR_ARG;
+  *n = c->nsim_batch;
+  return SGDLM_OK;
+}
+
+int sgdlm_init_sim_memory(sgdlm_ctx *c, size_t nsim, size_t nsim_batch) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  if (nsim < 1)
+    return fail(c, SGDLM_ERR_ARG, "nsim must be positive");
+  if (nsim >= 0x80000000ull)
+    return fail(c, SGDLM_ERR_ARG, "nsim=%zu too large", nsim);
+  (void)hipStreamSynchronize(c->stream);
+  free_pool(c->sim);
+  c->nsim = nsim;
+  c->nsim_batch = nsim_batch > nsim ? nsim : nsim_batch;
+  c->vb_valid = false;
+  c->sim_memory_initialized = true;
+  if (c->parents_set)
+    return ensure_sim_buffers(c);
+  return SGDLM_OK;
+}
+
+int sgdlm_clear_sim_memory(sgdlm_ctx *c) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  (void)hipStreamSynchronize(c->stream);
+  free_pool(c->sim);
+  c->sim_memory_initialized = false;
+  c->vb_valid = false;
+  c->nsim = c->nsim_batch = 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_init_forecast_priors(sgdlm_ctx *c) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  const size_t m = c->m, mp = c->max_p;
+  HIPCHK(c, hipMemcpyAsync(M_<double>(c, "fn"), M_<double>(c, "n"), m * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M_<double>(c, "fs"), M_<double>(c, "s"), m * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M_<double>(c, "fm"), M_<double>(c, "m"), m * mp * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M_<double>(c, "fC"), M_<double>(c, "C"), m * mp * mp * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(M_<double>(c, "fW"), M_<double>(c, "W"), m * mp * mp * 8, hipMemcpyDeviceToDevice, c->stream));
+  return sync_and_collect(c);
+}
+
+int sgdlm_get_is_prior(const sgdlm_ctx *c, int *v) {
+  if (!c || !v)
+    return SGDLM_ERR_ARG;
+  *v = c->is_prior ? 1 : 0;
+  return SGDLM_OK;
+}
+int sgdlm_set_is_prior(sgdlm_ctx *c, int v) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  c->is_prior = v != 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_get_parameters(const sgdlm_ctx *cc, double *m_t, double *C_t,
+                         double *n_t, double *s_t) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  const size_t m = c->m, mp = c->max_p;
+  if (m_t && (rc = d2h(c, m_t, M_<double>(c, "m"), m * mp * 8)))
+    return rc;
+  if (C_t && (rc = d2h(c, C_t, M_<double>(c, "C"), m * mp * mp * 8)))
+    return rc;
+  if (n_t && (rc = d2h(c, n_t, M_<double>(c, "n"), m * 8)))
+    return rc;
+  if (s_t && (rc = d2h(c, s_t, M_<double>(c, "s"), m * 8)))
+    return rc;
+  return sync_and_collect(c);
+}
+
+int sgdlm_set_parameters(sgdlm_ctx *c, const double *m_t, const double *C_t,
+                         const double *n_t, const double *s_t) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  const size_t m = c->m, mp = c->max_p;
+  if (m_t && (rc = h2d(c, M_<double>(c, "m"), m_t, m * mp * 8)))
+    return rc;
+  if (C_t && (rc = h2d(c, M_<double>(c, "C"), C_t, m * mp * mp * 8)))
+    return rc;
+  if (n_t && (rc = h2d(c, M_<double>(c, "n"), n_t, m * 8)))
+    return rc;
+  if (s_t && (rc = h2d(c, M_<double>(c, "s"), s_t, m * 8)))
+    return rc;
+  return sync_and_collect(c);
+}
+
+int sgdlm_get_discount_factors(const sgdlm_ctx *cc, double *beta, double *delta) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  if (beta && (rc = d2h(c, beta, M_<double>(c, "beta"), c->m * 8)))
+    return rc;
+  if (delta && (rc = d2h(c, delta, M_<double>(c, "delta"), c->m * c->max_p * c->max_p * 8)))
+    return rc;
+  return sync_and_collect(c);
+}
+
+int sgdlm_set_discount_factors(sgdlm_ctx *c, const double *beta, const double *delta) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  if (beta && (rc = h2d(c, M_<double>(c, "beta"), beta, c->m * 8)))
+    return rc;
+  if (delta && (rc = h2d(c, M_<double>(c, "delta"), delta, c->m * c->max_p * c->max_p * 8)))
+    return rc;
+  return sync_and_collect(c);
+}
+
+int sgdlm_get_evolution_matrix(const sgdlm_ctx *cc, double *G) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_evo(c)))
+    return rc;
+  if (G && (rc = d2h(c, G, M_<double>(c, "G"), c->m * c->max_p * c->max_p * 8)))
+    return rc;
+  return sync_and_collect(c);
+}
+
+int sgdlm_set_evolution_matrix(sgdlm_ctx *c, const double *G) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_evo(c)))
+    return rc;
+  if (G && (rc = h2d(c, M_<double>(c, "G"), G, c->m * c->max_p * c->max_p * 8)))
+    return rc;
+  return sync_and_collect(c);
+}
+
+int sgdlm_get_parental_sets(const sgdlm_ctx *c, unsigned int *p, unsigned int *sp) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (!c->memory_initialized)
+    return SGDLM_ERR_STATE;
+  if (p)
+    memcpy(p, c->h_p.data(), c->m * 4);
+  if (sp)
+    memcpy(sp, c->h_sp.data(), c->m * c->max_p * 4);
+  return SGDLM_OK;
+}
+
+int sgdlm_set_parental_sets(sgdlm_ctx *c, const unsigned int *p, const unsigned int *sp) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc = check_init(c);
+  if (rc)
+    return rc;
+  const size_t m = c->m, mp = c->max_p;
+  std::vector<unsigned> np = c->h_p, nsp = c->h_sp;
+  if (p)
+    np.assign(p, p + m);
+  if (sp)
+    nsp.assign(sp, sp + m * mp);
+  const unsigned long long mm = (unsigned long long)m * m;
+  std::vector<char> used(mm, 0);
+  for (size_t j = 0; j < m; ++j) {
+    if (np[j] > mp)
+      return fail(c, SGDLM_ERR_ARG, "p[%zu]=%u exceeds max_p=%zu", j, np[j], mp);
+    for (size_t k = 0; k < mp; ++k) {
+      const unsigned v = nsp[j * mp + k];
+      if (v >= mm)
+        continue;
+      if (k >= np[j])
+        return fail(c, SGDLM_ERR_ARG, "series %zu: simultaneous parent position at k=%zu >= p=%u", j, k, np[j]);
+      if (v % m == v / m)
+        return fail(c, SGDLM_ERR_ARG, "series %zu: position %u is on the diagonal of Gamma", j, v);
+      if (used[v])
+        return fail(c, SGDLM_ERR_ARG, "position %u of Gamma is used twice", v);
+      used[v] = 1;
+    }
+  }
+  c->h_p = np;
+  c->h_sp = nsp;
+  if ((rc = h2d(c, M_<unsigned>(c, "p"), np.data(), m * 4)) ||
+      (rc = h2d(c, M_<unsigned>(c, "sp"), nsp.data(), m * mp * 4)))
+    return rc;
+  if ((rc = build_structure(c)))
+    return rc;
+  c->parents_set = true;
+  c->vb_valid = false;
+  if (c->sim_memory_initialized)
+    return ensure_sim_buffers(c);
+  return SGDLM_OK;
+}
+
+int sgdlm_compute_prior(sgdlm_ctx *c, const double *G, int evolve_forecast_variables) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_parents(c)))
+    return rc;
+  if (G && (rc = check_evo(c)))
+    return rc;
+  if (!evolve_forecast_variables && (rc = check_prior(c, false)))
+    return rc;
+  if (G && (rc = h2d(c, M_<double>(c, "G"), G, c->m * c->max_p * c->max_p * 8)))
+    return rc;
+  const bool f = evolve_forecast_variables != 0;
+  {
+    StageTimer t(c, "filter");
+    launch_prior(c->S, M_<double>(c, f ? "fm" : "m"), M_<double>(c, f ? "fC" : "C"),
+                 M_<double>(c, f ? "fn" : "n"), M_<double>(c, f ? "fW" : "W"),
+                 M_<double>(c, "beta"), M_<double>(c, "delta"),
+                 G ? M_<double>(c, "G") : nullptr, c->stream);
+  }
+  CHECK_LAUNCH(c);
+  if ((rc = sync_and_collect(c)))
+    return rc;
+  if (!f)
+    c->is_prior = true;
+  return SGDLM_OK;
+}
+
+int sgdlm_compute_posterior(sgdlm_ctx *c, const double *y, const double *F) {
+  if (!c || !y || !F)
+    return c ? fail(c, SGDLM_ERR_ARG, "null y or F") : SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_parents(c)) || (rc = check_prior(c, true)))
+    return rc;
+  if ((rc = h2d(c, M_<double>(c, "y"), y, c->m * 8)) ||
+      (rc = h2d(c, M_<double>(c, "F"), F, c->m * c->max_p * 8)))
+    return rc;
+  {
+    StageTimer t(c, "filter");
+    launch_posterior(c->S, M_<double>(c, "m"), M_<double>(c, "C"), M_<double>(c, "n"),
+                     M_<double>(c, "s"), M_<double>(c, "y"), M_<double>(c, "F"), c->stream);
+  }
+  CHECK_LAUNCH(c);
+  if ((rc = sync_and_collect(c)))
+    return rc;
+  c->is_prior = false;
+  return SGDLM_OK;
+}
+
+int sgdlm_compute_vb_posterior(sgdlm_ctx *c, double *mean_m, double *mean_C,
+                               double *mean_n, double *mean_s, double *w,
+                               double *sum_det) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_parents(c)) || (rc = check_sim(c)) ||
+      (rc = check_prior(c, false)))
+    return rc;
+  if ((rc = run_vb(c)))
+    return rc;
+  const bool any = mean_m || mean_C || mean_n || mean_s || w || sum_det;
+  if (!any)
+    return sync_and_collect(c);
+  double stats[8];
+  rc = sgdlm_get_vb_result(c, mean_m, mean_C, mean_n, mean_s, w, nullptr, nullptr, nullptr);
+  if (rc)
+    return rc;
+  if ((rc = d2h(c, stats, SIM<double>(c, "stats"), 4 * 8)) || (rc = sync_and_collect(c)))
+    return rc;
+  if (sum_det)
+    *sum_det = std::exp(stats[0]) * stats[1];
+  if (!(stats[1] > 0.0))
+    return fail(c, SGDLM_ERR_NUMERIC, "every importance weight is zero (all |det(I-Gamma)| zero or non-finite)");
+  return SGDLM_OK;
+}
+
+int sgdlm_get_vb_result(const sgdlm_ctx *cc, double *mean_m, double *mean_C,
+                        double *mean_n, double *mean_s, double *w,
+                        double *log_w, double *ess, double *log_sum_det) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (!c->vb_valid)
+    return fail(c, SGDLM_ERR_STATE, "no VB result on the device");
+  const size_t m = c->m, mp = c->max_p, N = c->nsim;
+  int rc;
+  if (mean_m && (rc = d2h(c, mean_m, SIM<double>(c, "mean_m"), m * mp * 8)))
+    return rc;
+  if (mean_C && (rc = d2h(c, mean_C, SIM<double>(c, "mean_C"), m * mp * mp * 8)))
+    return rc;
+  if (mean_n && (rc = d2h(c, mean_n, SIM<double>(c, "mean_n"), m * 8)))
+    return rc;
+  if (mean_s && (rc = d2h(c, mean_s, SIM<double>(c, "mean_s"), m * 8)))
+    return rc;
+  if (w && (rc = d2h(c, w, SIM<double>(c, "w"), N * 8)))
+    return rc;
+  if (log_w && (rc = d2h(c, log_w, SIM<double>(c, "logdet"), N * 8)))
+    return rc;
+  double stats[8];
+  if ((rc = d2h(c, stats, SIM<double>(c, "stats"), 4 * 8)) || (rc = sync_and_collect(c)))
+    return rc;
+  if (log_w)
+    for (size_t i = 0; i < N; ++i)
+      if (!std::isfinite(log_w[i]))
+        log_w[i] = -INFINITY;
+  if (ess)
+    *ess = 1.0 / stats[2];
+  if (log_sum_det)
+    *log_sum_det = stats[0] + std::log(stats[1]);
+  return SGDLM_OK;
+}
+
+int sgdlm_evolve_forecast_samples(sgdlm_ctx *c, const double *G) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_parents(c)) || (rc = check_sim(c)))
+    return rc;
+  if (G && (rc = check_evo(c)))
+    return rc;
+  if ((rc = ensure_sim_buffers(c)))
+    return rc;
+  if (G && (rc = h2d(c, M_<double>(c, "G"), G, c->m * c->max_p * c->max_p * 8)))
+    return rc;
+  const uint32_t ep = ++c->epoch;
+  {
+    StageTimer t(c, "evolve");
+    launch_cholesky(c->S, M_<double>(c, "fW"), M_<double>(c, "U"), c->stream);
+    launch_evolve(c->S, (long)c->nsim, M_<double>(c, "U"), M_<double>(c, "fn"),
+                  M_<double>(c, "fs"), M_<double>(c, "beta"),
+                  G ? M_<double>(c, "G") : nullptr, M_<double>(c, "fm"), c->seed, ep,
+                  SIM<double>(c, "lam"), SIM<double>(c, "theta"), c->stream);
+  }
+  CHECK_LAUNCH(c);
+  c->vb_valid = false;
+  return sync_and_collect(c);
+}
+
+int sgdlm_compute_forecast(sgdlm_ctx *c, double *y, const double *x,
+                           double *lambdas, double *thetas, int use_existing) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_parents(c)) || (rc = check_sim(c)) ||
+      (rc = check_prior(c, true)))
+    return rc;
+  if (!x)
+    return fail(c, SGDLM_ERR_ARG, "null x_tp1");
+  if ((rc = ensure_sim_buffers(c)))
+    return rc;
+  const size_t m = c->m, mp = c->max_p, N = c->nsim;
+  if ((rc = dalloc(c, c->sim, "rhs", N * m * 8)) || (rc = dalloc(c, c->sim, "yf", N * m * 8)))
+    return rc;
+  if ((rc = h2d(c, M_<double>(c, "x"), x, m * mp * 8)))
+    return rc;
+  if (!use_existing) {
+    const uint32_t ep = ++c->epoch;
+    StageTimer t(c, "draw");
+    launch_cholesky(c->S, M_<double>(c, "fC"), M_<double>(c, "U"), c->stream);
+    launch_draw(c->S, (long)N, M_<double>(c, "fm"), M_<double>(c, "U"),
+                M_<double>(c, "fn"), M_<double>(c, "fs"), c->seed, ep,
+                SIM<double>(c, "lam"), SIM<double>(c, "theta"), c->stream);
+  }
+  CHECK_LAUNCH(c);
+  const uint32_t ep_nu = ++c->epoch;
+  launch_forecast_rhs(c->S, (long)N, M_<double>(c, "x"), M_<double>(c, "fm"),
+                      SIM<double>(c, "theta"), SIM<double>(c, "lam"), c->seed, ep_nu,
+                      SIM<double>(c, "rhs"), c->stream);
+  CHECK_LAUNCH(c);
+  const int prec = (c->prec == SGDLM_PREC_FP64) ? SGDLM_PREC_FP64 : SGDLM_PREC_FP32;
+  LuPlan plan;
+  if ((rc = lu_workspace(c, prec, 1, &plan)))
+    return rc;
+  {
+    StageTimer t(c, "forecast_lu");
+    rc = launch_lu_solve(c->S, plan, (long)N, SIM<double>(c, "theta"), c->sim["lu_work"].ptr,
+                         SIM<double>(c, "rhs"), SIM<double>(c, "yf"),
+                         prec == SGDLM_PREC_FP64 ? 0 : 1, c->stream);
+  }
+  if (rc)
+    return fail(c, rc, "forecast LU launch failed");
+  c->vb_valid = false;
+  if (y && (rc = d2h(c, y, SIM<double>(c, "yf"), N * m * 8)))
+    return rc;
+  if (lambdas && (rc = d2h(c, lambdas, SIM<double>(c, "lam"), N * m * 8)))
+    return rc;
+  if (thetas) {
+    /* unpack in slices to bound the extra device memory */
+    const long slice = std::max<long>(1, (long)((256ull << 20) / (m * mp * 8)));
+    if ((rc = dalloc(c, c->sim, "theta_full", (size_t)std::min<long>(slice, (long)N) * m * mp * 8)))
+      return rc;
+    for (long i0 = 0; i0 < (long)N; i0 += slice) {
+      const long n = std::min<long>(slice, (long)N - i0);
+      launch_unpack_theta(c->S, n, SIM<double>(c, "theta") + (size_t)i0 * c->S.P,
+                          M_<double>(c, "fm"), SIM<double>(c, "theta_full"), c->stream);
+      CHECK_LAUNCH(c);
+      if ((rc = d2h(c, thetas + (size_t)i0 * m * mp, SIM<double>(c, "theta_full"), n * m * mp * 8)))
+        return rc;
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+  }
+  return sync_and_collect(c);
+}
+
+int sgdlm_set_precision(sgdlm_ctx *c, int prec) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (prec != SGDLM_PREC_FP64 && prec != SGDLM_PREC_FP32 && prec != SGDLM_PREC_BF16X3)
+    return fail(c, SGDLM_ERR_ARG, "unknown precision %d", prec);
+  c->prec = prec;
+  return SGDLM_OK;
+}
+int sgdlm_get_precision(const sgdlm_ctx *c, int *prec) {
+  if (!c || !prec)
+    return SGDLM_ERR_ARG;
+  *prec = c->prec;
+  return SGDLM_OK;
+}
+int sgdlm_set_rng_state(sgdlm_ctx *c, uint64_t seed, uint32_t epoch) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (epoch >= (1u << 24))
+    return fail(c, SGDLM_ERR_ARG, "epoch must be below 2^24");
+  c->seed = seed;
+  c->epoch = epoch;
+  return SGDLM_OK;
+}
+int sgdlm_get_rng_state(const sgdlm_ctx *c, uint64_t *seed, uint32_t *epoch) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (seed)
+    *seed = c->seed;
+  if (epoch)
+    *epoch = c->epoch;
+  return SGDLM_OK;
+}
+int sgdlm_set_timing(sgdlm_ctx *c, int enable) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  c->timing = enable != 0;
+  c->times.clear();
+  return SGDLM_OK;
+}
+int sgdlm_get_timing(const sgdlm_ctx *c, const char *stage, double *ms, long *launches) {
+  if (!c || !stage)
+    return SGDLM_ERR_ARG;
+  auto it = c->times.find(stage);
+  if (ms)
+    *ms = it == c->times.end() ? 0.0 : it->second.first;
+  if (launches)
+    *launches = it == c->times.end() ? 0 : it->second.second;
+  return SGDLM_OK;
+}
+int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (!c->parents_set)
+    return SGDLM_ERR_STATE;
+  LuPlan plan;
+  int rc = lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, &plan);
+  if (rc)
+    return rc;
+  if (grid)
+    *grid = plan.grid;
+  if (lds)
+    *lds = plan.lds;
+  if (nb)
+    *nb = plan.nb;
+  return SGDLM_OK;
+}
+
+} // extern "C"

@@ -64,7 +64,8 @@ struct LuPlan {
   size_t lds = 0;             /* dynamic LDS bytes                          */
   size_t work_bytes = 0;      /* global workspace bytes (grid * M * M * T)  */
 };
-int lu_plan(const Structure &S, int prec, int solve, int n_cu, LuPlan *plan);
+int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
+            LuPlan *plan);
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
                      const double *theta, void *work, double *logdet,
                      hipStream_t st);

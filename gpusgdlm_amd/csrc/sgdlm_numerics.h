@@ -368,11 +368,13 @@ SG_HD void sg_cholesky_upper(int p, int max_p, int ld, const double *C,
 /* One naive-posterior draw of series j: lambda ~ Gamma(n/2, n s/2) and
  * theta_k = m_k + (U^T z)_k / sqrt(s lambda) for k < p, with z ~ N(0, I_p)
  * (SGDLM.cu:638-673).  U is the upper Cholesky factor of C in the layout of
- * sg_cholesky_upper.  Writes theta[0..p). */
+ * sg_cholesky_upper.  Writes theta[k * ts], k < p (ts = element stride, so a
+ * caller may keep the accumulators in strided scratch memory; the arithmetic
+ * does not depend on it). */
 SG_HD double sg_draw_lambda_theta(int p, int ld, const double *mj,
                                   const double *Uj, double n, double s,
                                   uint64_t seed, uint32_t epoch, uint32_t draw,
-                                  uint32_t series, double *theta) {
+                                  uint32_t series, double *theta, int ts) {
   double lam = sg_gamma(0.5 * n, 0.5 * n * s, seed, epoch, SG_PURPOSE_LAMBDA,
                         draw, series);
   double scale = __builtin_sqrt(s * lam);
@@ -380,20 +382,21 @@ SG_HD double sg_draw_lambda_theta(int p, int ld, const double *mj,
   /* z_k for k = 2q, 2q+1 come from slot q; compute the transform column by
    * column: theta_k = sum_{q<=k} U[k*ld+q] z_q. */
   for (k = 0; k < p; ++k)
-    theta[k] = 0.0;
+    theta[k * ts] = 0.0;
   for (q = 0; q < p; q += 2) {
     double z0, z1;
     sg_normal_pair(sg_rng_block(seed, epoch, SG_PURPOSE_THETA, draw, series,
                                 (uint32_t)(q >> 1)),
                    &z0, &z1);
     for (k = q; k < p; ++k) {
-      theta[k] = theta[k] + Uj[k * ld + q] * z0;
+      double t = theta[k * ts] + Uj[k * ld + q] * z0;
       if (q + 1 <= k)
-        theta[k] = theta[k] + Uj[k * ld + q + 1] * z1;
+        t = t + Uj[k * ld + q + 1] * z1;
+      theta[k * ts] = t;
     }
   }
   for (k = 0; k < p; ++k)
-    theta[k] = mj[k] + theta[k] / scale;
+    theta[k * ts] = mj[k] + theta[k * ts] / scale;
   return lam;
 }
 

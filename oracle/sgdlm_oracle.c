@@ -183,7 +183,7 @@ void sgo_sample(int m, int max_p, const unsigned *p, const double *m_t,
       int k;
       lambdas[(size_t)i * m + j] = sg_draw_lambda_theta(
           P, max_p, mt, U + (size_t)j * mp2, n_t[j], s_t[j], seed, epoch,
-          (uint32_t)i, (uint32_t)j, th);
+          (uint32_t)i, (uint32_t)j, th, 1);
       for (k = P; k < max_p; ++k)
         th[k] = mt[k];
     }
@@ -322,12 +322,13 @@ double sgo_mean_n(unsigned p, double mean_Q, double mean_lambda,
   int iter = 0;
   double mn = n_t;
   double fun, dfun, step = 0.0;
-  const double pq = (double)p - mean_Q;
+  const double pd = (double)p;
   do {
     mn -= step;
-    fun = log(mn + pq) - sgo_psi(mn / 2.0) - pq / mn - log(2.0 * mean_lambda) +
-          mean_log_lambda;
-    dfun = 1.0 / (mn + pq) - 0.5 * sgo_psi1(mn / 2.0) + pq / (mn * mn);
+    fun = log(mn + pd - mean_Q) - sgo_psi(mn / 2.0) - (pd - mean_Q) / mn -
+          log(2.0 * mean_lambda) + mean_log_lambda;
+    dfun = 1.0 / (mn + pd - mean_Q) - 0.5 * sgo_psi1(mn / 2.0) +
+           (pd - mean_Q) / (mn * mn);
     step = fun / dfun;
     if (step > mn)
       step = 0.5 * mn;

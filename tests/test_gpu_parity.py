@@ -1,0 +1,158 @@
+"""GPU parity of the HIP path against the FP64 oracle (same Philox seeds and
+epochs on both sides).
+
+Bars (stated per test):
+  - filter / prior recurrences: 1e-12 relative (FP64, same operation order);
+  - lambda / theta draws: bit-exact;
+  - FP64 LU log-determinants: 1e-10 absolute; FP64 VB moments: 1e-10 relative;
+  - FP32-MFMA LU: VB moments within 1e-4 relative, ESS within 1%;
+  - BF16x3 LU: VB moments within 1e-3 relative, ESS within 1%;
+  - forecast draws y: FP64 1e-9 relative, FP32 + refinement 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+from gpusgdlm_amd import SGDLM
+from gpusgdlm_amd.workloads import configure, filter_steps, random_model, readme_model
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SEED = 1234
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / den)
+
+
+def make_pair(mod, prec="fp64"):
+    g = SGDLM(strict=True)
+    g.set_rng_state(SEED, 0)
+    g.set_precision(prec)
+    o = orc.OracleSGDLM(seed=SEED, epoch=0)
+    configure(g, mod)
+    configure(o, mod)
+    return g, o
+
+
+def assert_params_close(pg, po, tol):
+    for k in ("m", "C", "n", "s"):
+        e = relerr(pg[k], po[k])
+        assert e <= tol, f"{k}: rel err {e:.3e} > {tol}"
+
+
+def test_filter_recurrences_readme(hiplib):
+    mod = readme_model(T=80)
+    g, o = make_pair(mod)
+    for t in range(1, 60):
+        g.computePosterior(mod.y[t], mod.F(t))
+        o.computePosterior(mod.y[t], mod.F(t))
+        assert_params_close(g.getParameters(), o.getParameters(), 1e-12)
+        g.computePrior()
+        o.computePrior()
+        assert_params_close(g.getParameters(), o.getParameters(), 1e-12)
+
+
+def test_draws_bitexact_and_forecast_fp64(hiplib):
+    mod = readme_model(T=60)
+    g, o = make_pair(mod)
+    filter_steps(g, mod, 1, 40)
+    filter_steps(o, mod, 1, 40)
+    Fn = mod.F_forecast(40)
+    rg = g.computeForecastDebug(3000, 3000, Fn)
+    ro = o.computeForecastDebug(3000, 3000, Fn)
+    assert np.array_equal(rg["lambdas"], ro["lambdas"]), "lambda draws differ"
+    assert np.array_equal(rg["thetas"], ro["thetas"]), "theta draws differ"
+    assert relerr(rg["y"], ro["y"]) <= 1e-9
+
+
+def test_multistep_forecast_evolution(hiplib):
+    mod = readme_model(T=60)
+    g, o = make_pair(mod)
+    filter_steps(g, mod, 1, 30)
+    filter_steps(o, mod, 1, 30)
+    Fk = np.stack([mod.F_forecast(30 + k) for k in range(3)], axis=-1)
+    rg = g.computeForecastDebug(1000, 1000, Fk)
+    ro = o.computeForecastDebug(1000, 1000, Fk)
+    assert rg["y"].shape == (mod.m, 1000, 3)
+    assert np.array_equal(rg["lambdas"], ro["lambdas"])
+    assert np.array_equal(rg["thetas"], ro["thetas"])
+    assert relerr(rg["y"], ro["y"]) <= 1e-9
+
+
+def _vb_pair(mod, prec, N, steps):
+    g, o = make_pair(mod, prec)
+    filter_steps(g, mod, 1, steps)
+    filter_steps(o, mod, 1, steps)
+    g.computePosterior(mod.y[steps], mod.F(steps))
+    o.computePosterior(mod.y[steps], mod.F(steps))
+    vg = g.computeVBPosterior(N, N)
+    vo = o.computeVBPosterior(N, N, diag=True)
+    return g, o, vg, vo
+
+
+def test_vb_fp64_readme(hiplib):
+    mod = readme_model(T=60)
+    g, o, vg, vo = _vb_pair(mod, "fp64", 10000, 30)
+    res = g.get_vb_result()
+    assert abs(np.sum(vg["IS_weights"]) - 1.0) < 1e-12
+    assert relerr(vg["IS_weights"], vo["IS_weights"]) <= 1e-10
+    assert abs(vg["sum_det_weights"] / vo["sum_det_weights"] - 1) <= 1e-10
+    assert abs(res["ess"] / vo["ess"] - 1) <= 1e-10
+    for k in ("m", "C", "n", "s"):
+        assert relerr(vg[k], vo[k]) <= 1e-10, k
+
+
+def test_logdet_fp64_m400_vs_oracle(hiplib):
+    mod = random_model(400, 20, T=12, seed=7)
+    g, o = make_pair(mod)
+    filter_steps(g, mod, 1, 8)
+    filter_steps(o, mod, 1, 8)
+    g.computePosterior(mod.y[8], mod.F(8))
+    o.computePosterior(mod.y[8], mod.F(8))
+    N = 96
+    g.computeVBPosterior(N, N)
+    lw = g.get_vb_result()["log_weights"]
+    prm = o.getParameters()
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
+    assert np.max(np.abs(lw - ld)) <= 1e-10
+    # and one draw against LAPACK (numpy slogdet)
+    A = np.eye(mod.m)
+    spf = mod.sp_u32.ravel(order="F")
+    thf = th[:, :, 0].ravel(order="F")
+    for q, v in enumerate(spf):
+        if v < mod.m * mod.m:
+            A[v % mod.m, v // mod.m] = -thf[q]
+    assert abs(np.linalg.slogdet(A)[1] - lw[0]) <= 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-4), ("bf16x3", 1e-3)])
+def test_vb_mixed_precision_m100(hiplib, prec, tol):
+    mod = random_model(100, 5, T=40, seed=11)
+    g, o, vg, vo = _vb_pair(mod, prec, 4000, 30)
+    res = g.get_vb_result()
+    assert abs(res["ess"] / vo["ess"] - 1) <= 0.01
+    for k in ("m", "C", "n", "s"):
+        e = relerr(vg[k], vo[k])
+        assert e <= tol, f"{prec} {k}: {e:.3e}"
+
+
+def test_vb_fp64_m100(hiplib):
+    mod = random_model(100, 5, T=40, seed=11)
+    g, o, vg, vo = _vb_pair(mod, "fp64", 4000, 30)
+    for k in ("m", "C", "n", "s"):
+        assert relerr(vg[k], vo[k]) <= 1e-10, k
+    assert relerr(vg["IS_weights"], vo["IS_weights"]) <= 1e-10
+
+
+def test_forecast_fp32_refined_m100(hiplib):
+    mod = random_model(100, 5, T=40, seed=3)
+    g, o = make_pair(mod, "fp32")
+    filter_steps(g, mod, 1, 30)
+    filter_steps(o, mod, 1, 30)
+    yg = g.computeForecast(2000, 2000, mod.F_forecast(30))
+    yo = o.computeForecast(2000, 2000, mod.F_forecast(30))
+    assert relerr(yg, yo) <= 1e-8
