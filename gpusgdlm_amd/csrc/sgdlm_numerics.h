@@ -335,7 +335,7 @@ SG_HD double sg_normal_k(uint64_t seed, uint32_t epoch, uint32_t purpose,
 /* Upper Cholesky factor U (C = U^T U) of the p x p upper-left block of the
  * column-major max_p x max_p matrix C (leading dimension ld), written to U
  * with zeros outside the upper triangle of the block.  Operation order of
- * kernel_functions.cuh:844-902. */
+ * kernel_functions.cuh:844-902; a non-positive pivot gives a zero row. */
 SG_HD void sg_cholesky_upper(int p, int max_p, int ld, const double *C,
                              double *U) {
   int n, k, l;
@@ -351,8 +351,17 @@ SG_HD void sg_cholesky_upper(int p, int max_p, int ld, const double *C,
     for (k = 0; k < max_p; ++k)
       U[k * ld + n] = 0.0;
   for (n = 0; n < p; ++n) {
-    double diag = __builtin_sqrt(U[n * ld + n]);
-    double a;
+    double diag, a;
+    if (!(U[n * ld + n] > 0.0)) {
+      /* zero (or invalid) pivot of a positive semi-definite block, e.g. the
+       * evolution variance W = C (1/Delta - 1) with Delta = 1: the row of U
+       * is zero and contributes nothing (the reference computes 0 * (1/0) =
+       * NaN here). */
+      for (k = n; k < p; ++k)
+        U[k * ld + n] = 0.0;
+      continue;
+    }
+    diag = __builtin_sqrt(U[n * ld + n]);
     U[n * ld + n] = diag;
     a = 1.0 / diag;
     for (k = n + 1; k < p; ++k) {
