@@ -8,32 +8,38 @@
  * (SGDLM.cu:556-620), which the reference runs on cuBLAS with the matrices
  * round-tripping HBM.
  *
- * Design (one persistent workgroup of 4 wavefronts per CU, looping over
- * draws):
- *   - The matrix of a draw lives in a per-workgroup global workspace
- *     (row-major, M x M, M = m rounded up to 16, identity padding), hot in
- *     L2 / Infinity Cache.  Row interchanges are never applied to memory: a
- *     logical->physical row map in LDS is permuted instead.
- *   - Blocked right-looking LU with panel width NB.  The panel
- *     (rows K..M, columns K..K+NB) is factored in LDS with partial pivoting
- *     (the pivot search picks the largest |a|, lowest row on ties, like
- *     LAPACK's getf2).
- *   - U12 = L11^{-1} A12 is formed per 16-column tile on MFMA from the
- *     inverses of the 16 x 16 diagonal blocks of L11, and kept in registers
- *     in the accumulator layout; the contraction index of the trailing
- *     update is permuted so that this accumulator layout is directly the
- *     MFMA B-operand layout (no shuffles).
- *   - A22 -= L21 U12 on MFMA: f64 16x16x4 (FP64), f32 16x16x4 (FP32) or
- *     three bf16 16x16x32 products hi*hi + hi*lo + lo*hi with FP32
- *     accumulation (BF16x3).  Work items are (column tile, row tile) pairs,
- *     split into 4 contiguous ranges, so the waves stay balanced.
- *   - log|det| = sum log|u_kk| is accumulated in FP64 during the panel
- *     factorisation; a zero or non-finite pivot marks the draw singular
- *     (log|det| = -inf, weight 0, as detFromLU's NaN/inf -> 0).
- *   - Forecast mode additionally keeps L and U in the workspace, solves
- *     L U y = P rhs with FP64 accumulation and optionally applies one step
- *     of iterative refinement with the FP64 residual rhs - (I - Gamma) y
- *     evaluated on the sparse Gamma.
+ * Both kernels are persistent (one workgroup per CU, looping over draws);
+ * the matrix of a draw lives in a per-workgroup global workspace (row-major,
+ * M x M, M = m rounded up to 16 with identity padding) that stays hot in L2 /
+ * Infinity Cache, and is assembled there from the packed coefficient draws.
+ *
+ * k_lu_fast (VB log-determinants; 8 wavefronts):
+ *   Partial pivoting performs no row interchange at all exactly when every
+ *   multiplier satisfies |l_ij| <= 1, and then it is the LU without
+ *   interchanges.  I - Gamma draws almost always have that property, so this
+ *   kernel factors without interchanges (no pivot searches, one barrier per
+ *   column: the panel rows live in registers, the pivot row is broadcast
+ *   through LDS), tracks max |l_ij|, and hands any draw with max |l| > 1 or
+ *   a zero pivot to k_lu_pivot through a device-side list.  The result is
+ *   therefore the partial-pivoting factorisation for every draw.
+ *     - U12 = L11^{-1} A12 per 16-column tile on MFMA from the inverses of
+ *       the 16 x 16 diagonal blocks of L11 (written back in place of A12);
+ *     - A22 -= L21 U12 on MFMA: f64 16x16x4 (FP64), f32 16x16x4 (FP32), or
+ *       bf16 16x16x32 hi*hi + hi*lo + lo*hi with FP32 accumulation (BF16x3);
+ *       work items (column tile, row tile) are split into 8 contiguous
+ *       ranges and their loads are issued one batch ahead.
+ *   The contraction index of every MFMA is permuted so that an accumulator
+ *   tile can be reused directly as a B operand (rho() below).
+ * k_lu_pivot (fallback and forecast solves; 4 wavefronts):
+ *   the same blocked algorithm with partial pivoting in the panel (largest
+ *   |a|, lowest row on ties, as LAPACK getf2); interchanges permute a
+ *   logical->physical row map in LDS instead of moving rows.  Forecast mode
+ *   keeps L and U and solves L U y = P rhs with FP64 accumulation, then
+ *   optionally refines once with the FP64 residual rhs - (I - Gamma) y on
+ *   the sparse Gamma.
+ * log|det| = sum log|u_kk| is accumulated in FP64; a zero or non-finite
+ * pivot after pivoting gives log|det| = -inf (weight 0, as detFromLU's
+ * NaN/inf -> 0).
  */
 #include <hip/hip_runtime.h>
 
@@ -44,8 +50,6 @@
 
 namespace sgdlm {
 namespace {
-
-constexpr int WG = 256; /* 4 wavefronts */
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -68,6 +72,14 @@ __device__ __forceinline__ f32x4 mfma_bf16(u16x8 a, u16x8 b, f32x4 c) {
                                                  c, 0, 0, 0);
 }
 
+/* Row (within a 16 x 16 tile) of accumulator register r in lane group
+ * g = lane / 16; the column is lane % 16.  v_mfma_f32_16x16x4f32 and
+ * v_mfma_f32_16x16x32_bf16: row = 4g + r.  v_mfma_f64_16x16x4f64:
+ * row = 4r + g. */
+template <typename T> __device__ __forceinline__ int rho(int g, int r);
+template <> __device__ __forceinline__ int rho<float>(int g, int r) { return 4 * g + r; }
+template <> __device__ __forceinline__ int rho<double>(int g, int r) { return 4 * r + g; }
+
 /* float -> bf16 bits, round to nearest even (finite inputs). */
 __device__ __forceinline__ unsigned short f2bf(float x) {
   unsigned u = __float_as_uint(x);
@@ -80,25 +92,9 @@ __device__ __forceinline__ float bf2f(unsigned short h) {
 
 enum { MODE_NATIVE = 0, MODE_BF16X3 = 1 };
 
-template <typename T, int NB, int MODE> struct LuShape {
-  /* LDS row stride of the panel: odd for conflict-free column access in the
-   * panel factorisation; 16-byte aligned rows for the bf16 operand loads. */
-  static constexpr int PS = (MODE == MODE_BF16X3) ? NB + 4 : NB + 1;
-  static constexpr int NQ = NB / 16;
-};
-
-__host__ __device__ inline size_t lu_lds_bytes(int M, int NB, int PS,
-                                               size_t tsize, bool solve) {
-  size_t b = (size_t)M * PS * tsize;       /* panel                        */
-  b += (size_t)NB * 16 * tsize;            /* diagonal-block inverses      */
-  b += (size_t)M * sizeof(int);            /* row map                      */
-  b = (b + 15) & ~(size_t)15;
-  b += 4 * (sizeof(double) + sizeof(int)); /* pivot reduction              */
-  b = (b + 15) & ~(size_t)15;
-  if (solve)
-    b += (size_t)M * sizeof(double);       /* solve vector                 */
-  return b;
-}
+/* LDS row stride of the panel: odd for conflict-free column access; 16-byte
+ * aligned rows for the bf16 operand loads. */
+constexpr int panel_stride(int NB, int MODE) { return MODE == MODE_BF16X3 ? NB + 4 : NB + 1; }
 
 struct LuArgs {
   int m, M;
@@ -112,14 +108,335 @@ struct LuArgs {
   const double *rhs;
   double *y;
   int refine;
+  int *flag_count; /* fast: appends flagged draws; pivot: number to process */
+  int *flag_list;
 };
 
-template <typename T, int NB, int MODE, bool SOLVE>
-__global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
-  typedef LuShape<T, NB, MODE> S;
+template <typename T>
+__device__ void assemble(T *A, int M, const double *th, const LuArgs &a, int tid,
+                         int nthreads) {
+  const size_t nvec = (size_t)M * M * sizeof(T) / 16;
+  float4 *A4 = reinterpret_cast<float4 *>(A);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (size_t q = tid; q < nvec; q += nthreads)
+    A4[q] = z;
+  __syncthreads();
+  for (int r = tid; r < M; r += nthreads)
+    A[(size_t)r * M + r] = (T)1;
+  for (int e = tid; e < a.n_gam; e += nthreads)
+    A[(size_t)a.g_row[e] * M + a.g_col[e]] = (T)(-th[a.g_idx[e]]);
+  __syncthreads();
+}
+
+/* Inverses of the nq 16x16 unit-lower diagonal blocks of the panel rows
+ * 0..16nq (LDS, stride PS) into Linv[b][i][c]; wave b, lanes 0..15. */
+template <typename T, int PS>
+__device__ void diag_block_inverses(const T *Pn, T *Linv, int nq, int wave,
+                                    int nwaves, int lane) {
+  for (int b = wave; b < nq; b += nwaves) {
+    if (lane < 16) {
+      const int c = lane;
+      T x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        x[i] = (i == c) ? (T)1 : (T)0;
+#pragma unroll
+      for (int i = 1; i < 16; ++i) {
+        if (i > c) {
+          T acc = (T)0;
+#pragma unroll
+          for (int t = 0; t < i; ++t)
+            acc = fma(Pn[(16 * b + i) * PS + 16 * b + t], x[t], acc);
+          x[i] = -acc;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        Linv[b * 256 + i * 16 + c] = x[i];
+    }
+  }
+}
+
+/* ========================================================================
+ * Fast path: no row interchanges, verified afterwards.
+ * ====================================================================== */
+template <typename T, int NB, int MODE, int RPT>
+__global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
   typedef typename Vec4<T>::type V4;
-  constexpr int PS = S::PS;
-  constexpr int NQ = S::NQ;
+  constexpr int NT = 512;
+  constexpr int NW = NT / 64;
+  constexpr int PS = panel_stride(NB, MODE);
+  constexpr int NQ = NB / 16;
+  constexpr int NKB = (NQ + 1) / 2; /* bf16 k-blocks of 32 */
+  constexpr int BATCH = 4;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+
+  const int M = a.M;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int l16 = lane & 15;
+  const int g = lane >> 4;
+
+  T *Pn = reinterpret_cast<T *>(smem);
+  T *Linv = Pn + (size_t)M * PS;
+  T *prow = Linv + NB * 16; /* [2][NB] */
+  float *redf = reinterpret_cast<float *>(prow + 2 * NB);
+
+  T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * M * M;
+
+  for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
+    assemble<T>(A, M, a.theta + (size_t)d * a.P, a, tid, NT);
+    double logdet = 0.0;
+    int bad = 0;
+    T amax = (T)0;
+
+    for (int K = 0; K < M; K += NB) {
+      const int jb = min(NB, M - K);
+      const int nr = M - K;
+      const int nq = jb >> 4;
+
+      /* ---- panel rows into registers ---- */
+      T v[RPT][NB];
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) {
+        const int r = tid + NT * rr;
+        const T *src = A + (size_t)(K + r) * M + K;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          v[rr][c] = (r < nr && c < jb) ? src[c] : (T)0;
+      }
+
+      /* ---- factor without interchanges: one barrier per column ---- */
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < jb) {
+          T *pr = prow + (j & 1) * NB;
+          if (tid == j) {
+#pragma unroll
+            for (int c = j; c < NB; ++c)
+              pr[c] = v[0][c];
+          }
+          __syncthreads();
+          T dj = pr[j];
+          if (!(fabs(dj) > (T)0)) {
+            bad = 1;
+            dj = (T)1;
+          } else if (tid == 0) {
+            logdet += log(fabs((double)dj));
+          }
+#pragma unroll
+          for (int rr = 0; rr < RPT; ++rr) {
+            const int r = tid + NT * rr;
+            if (r > j && r < nr) {
+              const T l = v[rr][j] / dj;
+              v[rr][j] = l;
+              amax = fmax(amax, fabs(l));
+#pragma unroll
+              for (int c = j + 1; c < NB; ++c)
+                v[rr][c] = fma(-l, pr[c], v[rr][c]);
+            }
+          }
+        }
+      }
+
+      /* ---- panel to LDS (bf16 mode: rows >= jb as -L21 hi/lo) ---- */
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) {
+        const int r = tid + NT * rr;
+        if (r < nr) {
+          if (MODE == MODE_BF16X3 && r >= jb) {
+            unsigned short *row = reinterpret_cast<unsigned short *>(Pn + r * PS);
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+              for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                  const int col = 32 * kb + 16 * (e >> 2) + 4 * gg + (e & 3);
+                  const float x = (col < NB) ? -(float)v[rr][col < NB ? col : 0] : 0.f;
+                  const unsigned short hi = f2bf(x);
+                  row[(kb * 4 + gg) * 16 + e] = hi;
+                  row[(kb * 4 + gg) * 16 + 8 + e] = f2bf(x - bf2f(hi));
+                }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+              Pn[r * PS + c] = v[rr][c];
+          }
+        }
+      }
+      __syncthreads();
+      diag_block_inverses<T, PS>(Pn, Linv, nq, wave, NW, lane);
+      __syncthreads();
+
+      const int ntc = (M - K - jb) >> 4;
+      if (ntc > 0) {
+        /* ---- phase A: U12 = L11^{-1} A12, written in place of A12 ---- */
+        for (int ct = wave; ct < ntc; ct += NW) {
+          const int c0 = K + jb + 16 * ct;
+          V4 u[NQ];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            V4 acc = {0, 0, 0, 0};
+            if (q < nq) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                acc[r] = A[(size_t)(K + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+            }
+            u[q] = acc;
+          }
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            if (q < nq) {
+              V4 acc = u[q];
+#pragma unroll
+              for (int aa = 0; aa < NQ; ++aa)
+                if (aa < q)
+#pragma unroll
+                  for (int r = 0; r < 4; ++r)
+                    acc = mfma16(-Pn[(16 * q + l16) * PS + 16 * aa + rho<T>(g, r)], u[aa][r], acc);
+              V4 uq = {0, 0, 0, 0};
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                uq = mfma16(Linv[q * 256 + l16 * 16 + rho<T>(g, r)], acc[r], uq);
+              u[q] = uq;
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                A[(size_t)(K + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[q][r];
+            }
+          }
+        }
+        __syncthreads();
+
+        /* ---- phase B: A22 -= L21 U12, balanced (ct, rt) items ---- */
+        const long tot = (long)ntc * ntc;
+        const long lo_it = tot * wave / NW, hi_it = tot * (wave + 1) / NW;
+        int cur_ct = -1;
+        V4 ub[NQ];
+        u16x8 bhi[NKB], blo[NKB];
+        V4 cc[BATCH], cn[BATCH];
+        auto load_batch = [&](long it0, V4 *dst) {
+#pragma unroll
+          for (int b = 0; b < BATCH; ++b) {
+            const long it = it0 + b;
+            if (it < hi_it) {
+              const int ct = (int)(it / ntc);
+              const int rt = (int)(it - (long)ct * ntc);
+              const T *src = A + (size_t)(K + jb + 16 * rt) * M + K + jb + 16 * ct + l16;
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                dst[b][r] = src[(size_t)rho<T>(g, r) * M];
+            }
+          }
+        };
+        if (lo_it < hi_it)
+          load_batch(lo_it, cc);
+        for (long it0 = lo_it; it0 < hi_it; it0 += BATCH) {
+          if (it0 + BATCH < hi_it)
+            load_batch(it0 + BATCH, cn);
+#pragma unroll
+          for (int b = 0; b < BATCH; ++b) {
+            const long it = it0 + b;
+            if (it < hi_it) {
+              const int ct = (int)(it / ntc);
+              const int rt = (int)(it - (long)ct * ntc);
+              const int c0 = K + jb + 16 * ct;
+              if (ct != cur_ct) {
+                cur_ct = ct;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                  V4 t = {0, 0, 0, 0};
+                  if (q < nq) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                      t[r] = A[(size_t)(K + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+                  }
+                  ub[q] = t;
+                }
+                if constexpr (MODE == MODE_BF16X3) {
+#pragma unroll
+                  for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                      const int q = 2 * kb + (e >> 2);
+                      const float x = (q < NQ) ? (float)ub[q < NQ ? q : 0][e & 3] : 0.f;
+                      const unsigned short h = f2bf(x);
+                      bhi[kb][e] = h;
+                      blo[kb][e] = f2bf(x - bf2f(h));
+                    }
+                }
+              }
+              V4 c = cc[b];
+              const int prw = jb + 16 * rt + l16;
+              if constexpr (MODE == MODE_BF16X3) {
+                const char *rowp = reinterpret_cast<const char *>(Pn + prw * PS);
+#pragma unroll
+                for (int kb = 0; kb < NKB; ++kb) {
+                  if (2 * kb < nq) {
+                    const u16x8 ahi = *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32);
+                    const u16x8 alo = *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32 + 16);
+                    c = mfma_bf16(alo, bhi[kb], c);
+                    c = mfma_bf16(ahi, blo[kb], c);
+                    c = mfma_bf16(ahi, bhi[kb], c);
+                  }
+                }
+              } else {
+#pragma unroll
+                for (int aa = 0; aa < NQ; ++aa)
+                  if (aa < nq)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                      c = mfma16(-Pn[prw * PS + 16 * aa + rho<T>(g, r)], ub[aa][r], c);
+              }
+              T *dst = A + (size_t)(K + jb + 16 * rt) * M + c0 + l16;
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                dst[(size_t)rho<T>(g, r) * M] = (T)c[r];
+            }
+          }
+#pragma unroll
+          for (int b = 0; b < BATCH; ++b)
+            cc[b] = cn[b];
+        }
+      }
+      __syncthreads();
+    }
+
+    /* ---- verify: any |l| > 1 or zero pivot -> partial-pivoting fallback ---- */
+    float wm = (float)amax;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+      wm = fmaxf(wm, __shfl_xor(wm, o));
+    if (lane == 0)
+      redf[wave] = wm;
+    __syncthreads();
+    if (tid == 0) {
+      float mx = redf[0];
+      for (int w = 1; w < NW; ++w)
+        mx = fmaxf(mx, redf[w]);
+      if (bad || !(mx <= 1.0f)) {
+        const int slot = atomicAdd(a.flag_count, 1);
+        a.flag_list[slot] = (int)d;
+        a.logdet[d] = NAN;
+      } else {
+        a.logdet[d] = logdet;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+/* ========================================================================
+ * Partial-pivoting path (fallback for flagged draws, forecast solves).
+ * ====================================================================== */
+template <typename T, int NB, int MODE, bool SOLVE>
+__global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a) {
+  typedef typename Vec4<T>::type V4;
+  constexpr int WG = 256;
+  constexpr int NW = 4;
+  constexpr int PS = panel_stride(NB, MODE_NATIVE);
+  constexpr int NQ = NB / 16;
   extern __shared__ __attribute__((aligned(16))) char smem[];
 
   const int M = a.M;
@@ -142,25 +459,14 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
   double *sv = reinterpret_cast<double *>(smem + off); /* SOLVE only */
 
   T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * M * M;
+  const long count = a.flag_list ? (long)*a.flag_count : a.n_draws;
 
-  for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
+  for (long q = blockIdx.x; q < count; q += gridDim.x) {
+    const long d = a.flag_list ? (long)a.flag_list[q] : q;
     const double *th = a.theta + (size_t)d * a.P;
-
-    /* ---- assemble A = I - Gamma_d (row-major, identity padding) ---- */
-    {
-      const size_t nvec = (size_t)M * M * sizeof(T) / 16;
-      float4 *A4 = reinterpret_cast<float4 *>(A);
-      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (size_t q = tid; q < nvec; q += WG)
-        A4[q] = z;
-    }
-    __syncthreads();
-    for (int r = tid; r < M; r += WG) {
-      A[(size_t)r * M + r] = (T)1;
+    assemble<T>(A, M, th, a, tid, WG);
+    for (int r = tid; r < M; r += WG)
       rowmap[r] = r;
-    }
-    for (int e = tid; e < a.n_gam; e += WG)
-      A[(size_t)a.g_row[e] * M + a.g_col[e]] = (T)(-th[a.g_idx[e]]);
     __syncthreads();
 
     double logdet = 0.0;
@@ -171,16 +477,13 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
       const int nr = M - K;
       const int nq = jb >> 4;
 
-      /* ---- 1. load panel (logical rows K..M, columns K..K+NB) ---- */
-      for (int q = tid; q < nr * NB; q += WG) {
-        const int r = q / NB;
-        const int c = q - r * NB;
-        Pn[r * PS + c] =
-            (c < jb) ? A[(size_t)rowmap[K + r] * M + K + c] : (T)0;
+      for (int e = tid; e < nr * NB; e += WG) {
+        const int r = e / NB;
+        const int c = e - r * NB;
+        Pn[r * PS + c] = (c < jb) ? A[(size_t)rowmap[K + r] * M + K + c] : (T)0;
       }
       __syncthreads();
 
-      /* ---- 2. panel factorisation with partial pivoting ---- */
       for (int j = 0; j < jb; ++j) {
         T bv = (T)-1;
         int bi = 0x7fffffff;
@@ -208,7 +511,7 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
         double best = redv[0];
         int piv = redi[0];
 #pragma unroll
-        for (int w = 1; w < 4; ++w) {
+        for (int w = 1; w < NW; ++w) {
           const double ov = redv[w];
           const int oi = redi[w];
           if (ov > best || (ov == best && oi < piv)) {
@@ -247,176 +550,66 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
         __syncthreads();
       }
 
-      /* ---- 3. inverses of the 16x16 unit-lower diagonal blocks of L11 ---- */
-      for (int b = wave; b < nq; b += 4) {
-        if (lane < 16) {
-          const int c = lane;
-          T x[16];
-#pragma unroll
-          for (int i = 0; i < 16; ++i)
-            x[i] = (i == c) ? (T)1 : (T)0;
-#pragma unroll
-          for (int i = 1; i < 16; ++i) {
-            if (i > c) {
-              T acc = (T)0;
-#pragma unroll
-              for (int t = 0; t < i; ++t)
-                acc = fma(Pn[(16 * b + i) * PS + 16 * b + t], x[t], acc);
-              x[i] = -acc;
-            }
-          }
-#pragma unroll
-          for (int i = 0; i < 16; ++i)
-            Linv[b * 256 + i * 16 + c] = x[i];
-        }
-      }
+      diag_block_inverses<T, PS>(Pn, Linv, nq, wave, NW, lane);
       if constexpr (SOLVE) {
-        /* keep L (below the diagonal) and U11 of the panel in the workspace */
-        for (int q = tid; q < nr * jb; q += WG) {
-          const int r = q / jb;
-          const int c = q - r * jb;
+        for (int e = tid; e < nr * jb; e += WG) {
+          const int r = e / jb;
+          const int c = e - r * jb;
           A[(size_t)rowmap[K + r] * M + K + c] = Pn[r * PS + c];
-        }
-      }
-      if constexpr (MODE == MODE_BF16X3) {
-        /* Split -L21 into bf16 hi/lo in place, in the permuted contraction
-         * order used below: chunk (kb, g) holds columns
-         * 32kb + 4g + {0..3} and 32kb + 16 + 4g + {0..3}. */
-        __syncthreads(); /* Linv reads rows < jb only, but keep it simple */
-        for (int r = jb + tid; r < nr; r += WG) {
-          float v[NB];
-#pragma unroll
-          for (int c = 0; c < NB; ++c)
-            v[c] = (float)Pn[r * PS + c];
-          unsigned short *row = reinterpret_cast<unsigned short *>(Pn + r * PS);
-#pragma unroll
-          for (int kb = 0; kb < NB / 32; ++kb)
-#pragma unroll
-            for (int gg = 0; gg < 4; ++gg) {
-#pragma unroll
-              for (int e = 0; e < 8; ++e) {
-                const int col = 32 * kb + 16 * (e >> 2) + 4 * gg + (e & 3);
-                const float x = -v[col];
-                const unsigned short hi = f2bf(x);
-                const unsigned short lo = f2bf(x - bf2f(hi));
-                row[(kb * 4 + gg) * 16 + e] = hi;
-                row[(kb * 4 + gg) * 16 + 8 + e] = lo;
-              }
-            }
         }
       }
       __syncthreads();
 
-      /* ---- 4. U12 and trailing update A22 -= L21 U12 on MFMA ---- */
+      /* U12 and the trailing update; each column tile is owned by one wave */
       const int ntc = (M - K - jb) >> 4;
-      if (ntc > 0) {
-        V4 uC[NQ];
-        u16x8 bhi[(NQ + 1) / 2], blo[(NQ + 1) / 2];
-        int cur_ct = -1;
-        long lo_it, hi_it;
-        if constexpr (SOLVE) {
-          lo_it = 0;
-          hi_it = 0; /* unused: SOLVE assigns whole column tiles per wave */
-        } else {
-          const long tot = (long)ntc * ntc;
-          lo_it = tot * wave / 4;
-          hi_it = tot * (wave + 1) / 4;
-        }
-        auto compute_u12 = [&](int ct) {
-          const int c0 = K + jb + 16 * ct;
+      for (int ct = wave; ct < ntc; ct += NW) {
+        const int c0 = K + jb + 16 * ct;
+        V4 u[NQ];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            V4 u = {0, 0, 0, 0};
-            if (q < nq) {
-              V4 acc;
+        for (int qq = 0; qq < NQ; ++qq) {
+          V4 acc = {0, 0, 0, 0};
+          if (qq < nq) {
 #pragma unroll
-              for (int r = 0; r < 4; ++r)
-                acc[r] = A[(size_t)rowmap[K + 16 * q + 4 * g + r] * M + c0 + l16];
+            for (int r = 0; r < 4; ++r)
+              acc[r] = A[(size_t)rowmap[K + 16 * qq + rho<T>(g, r)] * M + c0 + l16];
 #pragma unroll
-              for (int aa = 0; aa < NQ; ++aa)
-                if (aa < q)
-#pragma unroll
-                  for (int r = 0; r < 4; ++r)
-                    acc = mfma16(-Pn[(16 * q + l16) * PS + 16 * aa + 4 * g + r],
-                                 uC[aa][r], acc);
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-                u = mfma16(Linv[q * 256 + l16 * 16 + 4 * g + r], acc[r], u);
-              if constexpr (SOLVE) {
+            for (int aa = 0; aa < NQ; ++aa)
+              if (aa < qq)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                  A[(size_t)rowmap[K + 16 * q + 4 * g + r] * M + c0 + l16] = u[r];
-              }
+                  acc = mfma16(-Pn[(16 * qq + l16) * PS + 16 * aa + rho<T>(g, r)], u[aa][r], acc);
+            V4 uq = {0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              uq = mfma16(Linv[qq * 256 + l16 * 16 + rho<T>(g, r)], acc[r], uq);
+            acc = uq;
+            if constexpr (SOLVE) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                A[(size_t)rowmap[K + 16 * qq + rho<T>(g, r)] * M + c0 + l16] = acc[r];
             }
-            uC[q] = u;
           }
-          if constexpr (MODE == MODE_BF16X3) {
-#pragma unroll
-            for (int kb = 0; kb < (NQ + 1) / 2; ++kb)
-#pragma unroll
-              for (int e = 0; e < 8; ++e) {
-                const int q = 2 * kb + (e >> 2);
-                const float x = (q < NQ) ? (float)uC[q < NQ ? q : 0][e & 3] : 0.f;
-                const unsigned short h = f2bf(x);
-                bhi[kb][e] = h;
-                blo[kb][e] = f2bf(x - bf2f(h));
-              }
-          }
-        };
-        auto update_tile = [&](int ct, int rt) {
-          const int c0 = K + jb + 16 * ct;
+          u[qq] = acc;
+        }
+        for (int rt = 0; rt < ntc; ++rt) {
           const int row0 = K + jb + 16 * rt;
           int pr[4];
           V4 c;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            pr[r] = rowmap[row0 + 4 * g + r];
+            pr[r] = rowmap[row0 + rho<T>(g, r)];
             c[r] = A[(size_t)pr[r] * M + c0 + l16];
           }
-          const int prow = jb + 16 * rt + l16; /* panel row of this lane */
-          if constexpr (MODE == MODE_BF16X3) {
-            const char *rowp = reinterpret_cast<const char *>(Pn + prow * PS);
+          const int prw = jb + 16 * rt + l16;
 #pragma unroll
-            for (int kb = 0; kb < (NQ + 1) / 2; ++kb) {
-              if (2 * kb < nq) {
-                const u16x8 ahi =
-                    *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32);
-                const u16x8 alo = *reinterpret_cast<const u16x8 *>(
-                    rowp + (kb * 4 + g) * 32 + 16);
-                c = mfma_bf16(alo, bhi[kb], c);
-                c = mfma_bf16(ahi, blo[kb], c);
-                c = mfma_bf16(ahi, bhi[kb], c);
-              }
-            }
-          } else {
+          for (int aa = 0; aa < NQ; ++aa)
+            if (aa < nq)
 #pragma unroll
-            for (int aa = 0; aa < NQ; ++aa)
-              if (aa < nq)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                  c = mfma16(-Pn[prow * PS + 16 * aa + 4 * g + r], uC[aa][r], c);
-          }
+              for (int r = 0; r < 4; ++r)
+                c = mfma16(-Pn[prw * PS + 16 * aa + rho<T>(g, r)], u[aa][r], c);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            A[(size_t)pr[r] * M + c0 + l16] = (T)c[r];
-        };
-
-        if constexpr (SOLVE) {
-          for (int ct = wave; ct < ntc; ct += 4) {
-            compute_u12(ct);
-            for (int rt = 0; rt < ntc; ++rt)
-              update_tile(ct, rt);
-          }
-        } else {
-          for (long it = lo_it; it < hi_it; ++it) {
-            const int ct = (int)(it / ntc);
-            const int rt = (int)(it - (long)ct * ntc);
-            if (ct != cur_ct) {
-              compute_u12(ct);
-              cur_ct = ct;
-            }
-            update_tile(ct, rt);
-          }
+            A[(size_t)pr[r] * M + c0 + l16] = c[r];
         }
       }
       __syncthreads();
@@ -425,6 +618,7 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
     if constexpr (!SOLVE) {
       if (tid == 0)
         a.logdet[d] = singular ? -INFINITY : logdet;
+      __syncthreads();
       continue;
     }
 
@@ -432,7 +626,6 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
     const double *rhs = a.rhs + (size_t)d * a.m;
     double *yout = a.y + (size_t)d * a.m;
     for (int pass = 0; pass < (a.refine ? 2 : 1); ++pass) {
-      /* permuted right-hand side */
       for (int i = tid; i < M; i += WG) {
         const int src = rowmap[i];
         double b = 0.0;
@@ -440,7 +633,6 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
           if (pass == 0) {
             b = rhs[src];
           } else {
-            /* residual of row src: rhs - y + sum_e theta_e y_col(e) */
             double acc = rhs[src] - yout[src];
             for (int e = a.g_rowptr[src]; e < a.g_rowptr[src + 1]; ++e)
               acc += th[a.g_idx[e]] * yout[a.g_col[e]];
@@ -450,7 +642,6 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
         sv[i] = b;
       }
       __syncthreads();
-      /* forward substitution with unit-lower L (logical rows) */
       for (int kb = 0; kb < M / 16; ++kb) {
         const int b0 = 16 * kb;
         if (wave == 0 && lane < 16) {
@@ -474,7 +665,6 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
         }
         __syncthreads();
       }
-      /* back substitution with U */
       for (int kb = M / 16 - 1; kb >= 0; --kb) {
         const int b0 = 16 * kb;
         if (wave == 0 && lane < 16) {
@@ -502,46 +692,97 @@ __global__ __launch_bounds__(WG) void k_lu(LuArgs a) {
         __syncthreads();
       }
       for (int i = tid; i < a.m; i += WG) {
-        const double v = singular ? NAN : sv[i];
-        yout[i] = (pass == 0) ? v : yout[i] + v;
+        const double vv = singular ? NAN : sv[i];
+        yout[i] = (pass == 0) ? vv : yout[i] + vv;
       }
       __syncthreads();
     }
   }
 }
 
-template <typename T, int NB, int MODE, bool SOLVE>
-int launch_one(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
-  auto kern = k_lu<T, NB, MODE, SOLVE>;
-  if (hipFuncSetAttribute((const void *)kern,
-                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)plan.lds) != hipSuccess)
+size_t fast_lds_bytes(int M, int NB, int MODE, size_t ts) {
+  size_t b = (size_t)M * panel_stride(NB, MODE) * ts + (size_t)NB * 16 * ts + 2 * (size_t)NB * ts;
+  b = (b + 15) & ~(size_t)15;
+  return b + 16 * sizeof(float);
+}
+
+size_t pivot_lds_bytes(int M, int NB, size_t ts, bool solve) {
+  size_t b = (size_t)M * panel_stride(NB, MODE_NATIVE) * ts + (size_t)NB * 16 * ts + (size_t)M * sizeof(int);
+  b = (b + 15) & ~(size_t)15;
+  b += 4 * (sizeof(double) + sizeof(int));
+  b = (b + 15) & ~(size_t)15;
+  if (solve)
+    b += (size_t)M * sizeof(double);
+  return b;
+}
+
+template <typename Kern>
+int launch_kernel(Kern kern, size_t lds, int grid, int threads, const LuArgs &a, hipStream_t st) {
+  if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess)
     return SGDLM_ERR_DEVICE;
-  hipLaunchKernelGGL(kern, dim3(plan.grid), dim3(WG), plan.lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, a);
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 
-template <bool SOLVE>
-int dispatch(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
+template <typename T, int NB, int MODE>
+int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
+  if (a.M <= 512)
+    return launch_kernel(k_lu_fast<T, NB, MODE, 1>, plan.lds, plan.grid, 512, a, st);
+  if (a.M <= 1024)
+    return launch_kernel(k_lu_fast<T, NB, MODE, 2>, plan.lds, plan.grid, 512, a, st);
+  return SGDLM_ERR_ARG;
+}
+
+int dispatch_fast(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   switch (plan.prec) {
   case SGDLM_PREC_FP64:
     if (plan.nb == 32)
-      return launch_one<double, 32, MODE_NATIVE, SOLVE>(a, plan, st);
-    return launch_one<double, 16, MODE_NATIVE, SOLVE>(a, plan, st);
+      return fast_rpt<double, 32, MODE_NATIVE>(a, plan, st);
+    return fast_rpt<double, 16, MODE_NATIVE>(a, plan, st);
   case SGDLM_PREC_FP32:
     if (plan.nb == 64)
-      return launch_one<float, 64, MODE_NATIVE, SOLVE>(a, plan, st);
+      return fast_rpt<float, 64, MODE_NATIVE>(a, plan, st);
     if (plan.nb == 32)
-      return launch_one<float, 32, MODE_NATIVE, SOLVE>(a, plan, st);
-    return launch_one<float, 16, MODE_NATIVE, SOLVE>(a, plan, st);
+      return fast_rpt<float, 32, MODE_NATIVE>(a, plan, st);
+    return fast_rpt<float, 16, MODE_NATIVE>(a, plan, st);
   case SGDLM_PREC_BF16X3:
-    if (SOLVE)
-      return SGDLM_ERR_ARG;
     if (plan.nb == 64)
-      return launch_one<float, 64, MODE_BF16X3, false>(a, plan, st);
-    return launch_one<float, 32, MODE_BF16X3, false>(a, plan, st);
+      return fast_rpt<float, 64, MODE_BF16X3>(a, plan, st);
+    return fast_rpt<float, 32, MODE_BF16X3>(a, plan, st);
   }
   return SGDLM_ERR_ARG;
+}
+
+template <bool SOLVE>
+int dispatch_pivot(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
+  /* BF16x3 falls back to FP32 partial pivoting */
+  if (plan.prec == SGDLM_PREC_FP64) {
+    if (plan.pivot_nb == 32)
+      return launch_kernel(k_lu_pivot<double, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+    return launch_kernel(k_lu_pivot<double, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+  }
+  if (plan.pivot_nb == 64)
+    return launch_kernel(k_lu_pivot<float, 64, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+  if (plan.pivot_nb == 32)
+    return launch_kernel(k_lu_pivot<float, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+  return launch_kernel(k_lu_pivot<float, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+}
+
+LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
+  LuArgs a{};
+  a.m = S.m;
+  a.M = S.M;
+  a.n_draws = N;
+  a.theta = theta;
+  a.P = S.P;
+  a.n_gam = S.n_gam;
+  a.g_row = S.g_row;
+  a.g_col = S.g_col;
+  a.g_idx = S.g_idx;
+  a.g_rowptr = S.g_rowptr;
+  a.work = work;
+  return a;
 }
 
 } // namespace
@@ -550,25 +791,33 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             LuPlan *plan) {
   const int M = S.M;
   const size_t ts = (prec == SGDLM_PREC_FP64) ? 8 : 4;
-  int cands[3] = {64, 32, 16};
+  const int cands[3] = {64, 32, 16};
   plan->prec = prec;
   plan->nb = 0;
-  for (int i = 0; i < 3; ++i) {
+  plan->pivot_nb = 0;
+  for (int i = 0; i < 3 && !plan->nb; ++i) {
     const int nb = cands[i];
     if (prec == SGDLM_PREC_FP64 && nb > 32)
       continue;
     if (prec == SGDLM_PREC_BF16X3 && nb < 32)
       continue;
-    const int ps = (prec == SGDLM_PREC_BF16X3) ? nb + 4 : nb + 1;
-    const size_t lds = lu_lds_bytes(M, nb, ps, ts, solve != 0);
-    /* narrower panels only when the wide one does not fit */
-    if (lds <= lds_max) {
+    const size_t lds = fast_lds_bytes(M, nb, prec == SGDLM_PREC_BF16X3 ? MODE_BF16X3 : MODE_NATIVE, ts);
+    if (lds <= lds_max && M <= 1024) {
       plan->nb = nb;
       plan->lds = lds;
-      break;
     }
   }
-  if (plan->nb == 0)
+  for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
+    const int nb = cands[i];
+    if (prec == SGDLM_PREC_FP64 && nb > 32)
+      continue;
+    const size_t lds = pivot_lds_bytes(M, nb, ts, solve != 0);
+    if (lds <= lds_max) {
+      plan->pivot_nb = nb;
+      plan->pivot_lds = lds;
+    }
+  }
+  if (plan->pivot_nb == 0 || (!solve && plan->nb == 0))
     return SGDLM_ERR_ARG;
   plan->grid = n_cu > 0 ? n_cu : 256;
   plan->work_bytes = (size_t)plan->grid * M * M * ts;
@@ -577,42 +826,27 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
 
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
                      const double *theta, void *work, double *logdet,
-                     hipStream_t st) {
-  LuArgs a{};
-  a.m = S.m;
-  a.M = S.M;
-  a.n_draws = N;
-  a.theta = theta;
-  a.P = S.P;
-  a.n_gam = S.n_gam;
-  a.g_row = S.g_row;
-  a.g_col = S.g_col;
-  a.g_idx = S.g_idx;
-  a.g_rowptr = S.g_rowptr;
-  a.work = work;
+                     int *flags, hipStream_t st) {
+  LuArgs a = base_args(S, N, theta, work);
   a.logdet = logdet;
-  return dispatch<false>(a, plan, st);
+  a.flag_count = flags;
+  a.flag_list = flags + 1;
+  if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
+    return SGDLM_ERR_DEVICE;
+  int rc = dispatch_fast(a, plan, st);
+  if (rc)
+    return rc;
+  return dispatch_pivot<false>(a, plan, st);
 }
 
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
                     double *y, int refine, hipStream_t st) {
-  LuArgs a{};
-  a.m = S.m;
-  a.M = S.M;
-  a.n_draws = N;
-  a.theta = theta;
-  a.P = S.P;
-  a.n_gam = S.n_gam;
-  a.g_row = S.g_row;
-  a.g_col = S.g_col;
-  a.g_idx = S.g_idx;
-  a.g_rowptr = S.g_rowptr;
-  a.work = work;
+  LuArgs a = base_args(S, N, theta, work);
   a.rhs = rhs;
   a.y = y;
   a.refine = refine;
-  return dispatch<true>(a, plan, st);
+  return dispatch_pivot<true>(a, plan, st);
 }
 
 } // namespace sgdlm

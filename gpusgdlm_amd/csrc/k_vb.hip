@@ -251,7 +251,9 @@ __global__ __launch_bounds__(256) void k_mom2(long N, int m, int max_p,
       for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int e = (4 * g + r) * 16 + l16;
+          /* v_mfma_f64_16x16x4f64 accumulator: register r of lane group g
+           * holds row 4r + g, column lane % 16 */
+          const int e = (4 * r + g) * 16 + l16;
           tiles[t][e] = (wv == 0) ? acc[t][r] : tiles[t][e] + acc[t][r];
         }
     }

@@ -220,7 +220,8 @@ int ensure_sim_buffers(sgdlm_ctx *c) {
       (rc = dalloc(c, c->sim, "theta", N * std::max<size_t>(P, 1) * 8)) ||
       (rc = dalloc(c, c->sim, "logdet", N * 8)) ||
       (rc = dalloc(c, c->sim, "w", N * 8)) ||
-      (rc = dalloc(c, c->sim, "stats", 8 * 8)))
+      (rc = dalloc(c, c->sim, "stats", 8 * 8)) ||
+      (rc = dalloc(c, c->sim, "lu_flags", (N + 1) * 4)))
     return rc;
   const int chunks = vb_chunks((long)N);
   if ((rc = dalloc(c, c->sim, "part1", (size_t)chunks * m * (mp + 2) * 8)) ||
@@ -354,7 +355,7 @@ int run_vb(sgdlm_ctx *c) {
     StageTimer t(c, "lu");
     rc = launch_lu_logdet(c->S, plan, N, SIM<double>(c, "theta"),
                           c->sim["lu_work"].ptr, SIM<double>(c, "logdet"),
-                          c->stream);
+                          SIM<int>(c, "lu_flags"), c->stream);
   }
   if (rc)
     return fail(c, rc, "LU launch failed: %s", hipGetErrorString(hipGetLastError()));

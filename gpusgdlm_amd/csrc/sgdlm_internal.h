@@ -59,16 +59,20 @@ void launch_unpack_theta(const Structure &S, long N, const double *theta,
 /* ---- LU (k_lu.hip) ---- */
 struct LuPlan {
   int prec = SGDLM_PREC_FP64; /* element type / update path                */
-  int nb = 0;                 /* panel width                                */
+  int nb = 0;                 /* panel width, no-interchange fast kernel    */
+  int pivot_nb = 0;           /* panel width, partial-pivoting kernel       */
   int grid = 0;               /* persistent workgroups                      */
-  size_t lds = 0;             /* dynamic LDS bytes                          */
+  size_t lds = 0;             /* dynamic LDS bytes, fast kernel             */
+  size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
   size_t work_bytes = 0;      /* global workspace bytes (grid * M * M * T)  */
 };
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             LuPlan *plan);
+/* flags: N + 1 ints of scratch (count + list of draws handed to the
+ * partial-pivoting kernel). */
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
                      const double *theta, void *work, double *logdet,
-                     hipStream_t st);
+                     int *flags, hipStream_t st);
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
                     double *y, int refine, hipStream_t st);

@@ -156,3 +156,44 @@ def test_forecast_fp32_refined_m100(hiplib):
     yg = g.computeForecast(2000, 2000, mod.F_forecast(30))
     yo = o.computeForecast(2000, 2000, mod.F_forecast(30))
     assert relerr(yg, yo) <= 1e-8
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 2e-3)])
+def test_logdet_pivoting_fallback(hiplib, prec, tol):
+    """Posterior means of the simultaneous coefficients near 1.5 force row
+    interchanges (|l| > 1) in most draws: those draws go through the
+    partial-pivoting kernel and must still match the oracle's LU."""
+    mod = readme_model(T=30)
+    g, o = make_pair(mod, prec)
+    filter_steps(g, mod, 1, 10)
+    filter_steps(o, mod, 1, 10)
+    g.computePosterior(mod.y[10], mod.F(10))
+    o.computePosterior(mod.y[10], mod.F(10))
+    prm = o.getParameters()
+    mt = prm["m"].copy(order="F")
+    spf = mod.sp_u32
+    for j in range(mod.m):
+        for k in range(mod.max_p):
+            if spf[k, j] < mod.m * mod.m:
+                mt[k, j] = 1.5 if (j + k) % 2 else -1.3
+    for api in (g, o):
+        api.setPosteriorParameters(mt, prm["C"] * 0.01, prm["n"], prm["s"])
+    N = 2000
+    g.computeVBPosterior(N, N)
+    lw = g.get_vb_result()["log_weights"]
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, mt, prm["C"] * 0.01, prm["n"], prm["s"], N, SEED, 1)
+    ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
+    fin = np.isfinite(ld)
+    assert np.array_equal(np.isfinite(lw), fin)
+    assert np.max(np.abs(lw[fin] - ld[fin])) <= tol
+    # the oracle agrees that interchanges were needed for most draws
+    import scipy.linalg as sl
+    swaps = 0
+    for i in range(20):
+        A = np.eye(mod.m)
+        thf = th[:, :, i].ravel(order="F")
+        for q, v in enumerate(mod.sp_u32.ravel(order="F")):
+            if v < mod.m * mod.m:
+                A[v % mod.m, v // mod.m] = -thf[q]
+        swaps += int(np.any(sl.lu_factor(A)[1] != np.arange(mod.m)))
+    assert swaps >= 5
