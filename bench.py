@@ -55,6 +55,7 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter):
     stage = {s: g.get_timing(s) for s in ("draw", "lu", "weights", "moments")}
     g.set_timing(False)
     res = g.get_vb_result()
+    fallback = g.lu_fallback_count()
     # PCIe-inclusive figure through the full API (outputs copied to host)
     t1 = time.perf_counter()
     out = g.computeVBPosterior(N, N)
@@ -62,7 +63,7 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter):
     cfg = g.lu_config()
     g.close()
     return {"ms_per_step": 1e3 * dt / steps, "stage_ms": {k: v[0] / max(v[1], 1) for k, v in stage.items()},
-            "lu_launches": stage["lu"][1], "ess": res["ess"], "api_ms": 1e3 * api, "lu_config": cfg,
+            "lu_launches": stage["lu"][1], "ess": res["ess"], "lu_fallback_draws": fallback, "api_ms": 1e3 * api, "lu_config": cfg,
             "vb": out}
 
 
@@ -115,7 +116,8 @@ def main():
     for prec in a.precisions.split(","):
         log(f"[{prec}] {a.warmup} warm-up + {a.steps} timed VB steps, N={N}")
         results[prec] = run_gpu(mod, prec, N, a.steps, a.warmup, a.filter_steps)
-        log(f"[{prec}] {results[prec]['ms_per_step']:.2f} ms/step stages={results[prec]['stage_ms']}")
+        log(f"[{prec}] {results[prec]['ms_per_step']:.2f} ms/step stages={results[prec]['stage_ms']} "
+            f"fallback={results[prec]['lu_fallback_draws']}")
 
     head = "fp32" if "fp32" in results else next(iter(results))
     r = results[head]
@@ -141,6 +143,7 @@ def main():
         "is_draws_per_s": steps_s * N,
         "variants": {p: {"vb_steps_per_s": 1e3 / v["ms_per_step"], "is_draws_per_s": 1e3 / v["ms_per_step"] * N,
                          "ms_per_step": v["ms_per_step"], "stage_ms": v["stage_ms"], "ess": v["ess"],
+                         "lu_fallback_draws": v["lu_fallback_draws"],
                          "api_steps_per_s": 1e3 / v["api_ms"]}
                      for p, v in results.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",

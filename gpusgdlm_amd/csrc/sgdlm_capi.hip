@@ -1006,6 +1006,18 @@ int sgdlm_get_timing(const sgdlm_ctx *c, const char *stage, double *ms, long *la
     *launches = it == c->times.end() ? 0 : it->second.second;
   return SGDLM_OK;
 }
+int sgdlm_get_lu_fallback_count(const sgdlm_ctx *cc, long *count) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c || !count)
+    return SGDLM_ERR_ARG;
+  if (!c->vb_valid)
+    return fail(c, SGDLM_ERR_STATE, "no VB result on the device");
+  int n = 0;
+  HIPCHK(c, hipMemcpy(&n, SIM<int>(c, "lu_flags"), sizeof(int), hipMemcpyDeviceToHost));
+  *count = n;
+  return SGDLM_OK;
+}
+
 int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb) {
   sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
   if (!c)

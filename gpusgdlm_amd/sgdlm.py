@@ -342,6 +342,11 @@ class SGDLM:
         self._call("sgdlm_get_timing", stage.encode(), C.byref(ms), C.byref(n))
         return ms.value, n.value
 
+    def lu_fallback_count(self):
+        n = C.c_long()
+        self._call("sgdlm_get_lu_fallback_count", C.byref(n))
+        return n.value
+
     def lu_config(self):
         g, l, nb = C.c_int(), C.c_size_t(), C.c_int()
         self._call("sgdlm_get_lu_config", C.byref(g), C.byref(l), C.byref(nb))
