@@ -14,14 +14,15 @@
  * Infinity Cache, and is assembled there from the packed coefficient draws.
  *
  * k_lu_fast (VB log-determinants; 8 wavefronts):
- *   Partial pivoting performs no row interchange at all exactly when every
- *   multiplier satisfies |l_ij| <= 1, and then it is the LU without
- *   interchanges.  I - Gamma draws almost always have that property, so this
+ *   Threshold partial pivoting keeps the diagonal pivot while every
+ *   multiplier satisfies |l_ij| <= 1/tau (LAPACK's partial pivoting is
+ *   tau = 1 and then performs no interchange exactly when all |l_ij| <= 1).
+ *   For draws of I - Gamma the multipliers stay near or below 1, so this
  *   kernel factors without interchanges (no pivot searches, one barrier per
  *   column: the panel rows live in registers, the pivot row is broadcast
- *   through LDS), tracks max |l_ij|, and hands any draw with max |l| > 1 or
- *   a zero pivot to k_lu_pivot through a device-side list.  The result is
- *   therefore the partial-pivoting factorisation for every draw.
+ *   through LDS), tracks max |l_ij|, and hands any draw with
+ *   max |l| > 1/tau = 2 or a zero pivot to k_lu_pivot through a device-side
+ *   list.
  *     - U12 = L11^{-1} A12 per 16-column tile on MFMA from the inverses of
  *       the 16 x 16 diagonal blocks of L11 (written back in place of A12);
  *     - A22 -= L21 U12 on MFMA: f64 16x16x4 (FP64), f32 16x16x4 (FP32), or
@@ -56,9 +57,15 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef f32x4 type; };
 template <> struct Vec4<double> { typedef f64x4 type; };
+/* 16-byte vector of T */
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { typedef f32x4 type; static constexpr int n = 4; };
+template <> struct Vec16<double> { typedef f64x2 type; static constexpr int n = 2; };
 
 __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -91,6 +98,13 @@ __device__ __forceinline__ float bf2f(unsigned short h) {
 }
 
 enum { MODE_NATIVE = 0, MODE_BF16X3 = 1 };
+
+/* Threshold partial pivoting: the diagonal pivot of the fast kernel is
+ * accepted while every multiplier satisfies |l| <= 1 / tau with tau = 1/2
+ * (LAPACK's partial pivoting is tau = 1); element growth per step stays
+ * below 1 + 1/tau = 3.  Draws that exceed it are refactored with partial
+ * pivoting by k_lu_pivot. */
+constexpr float kMaxMultiplier = 2.0f;
 
 /* LDS row stride of the panel: odd for conflict-free column access; 16-byte
  * aligned rows for the bf16 operand loads. */
@@ -168,7 +182,9 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
   constexpr int PS = panel_stride(NB, MODE);
   constexpr int NQ = NB / 16;
   constexpr int NKB = (NQ + 1) / 2; /* bf16 k-blocks of 32 */
-  constexpr int BATCH = 4;
+  constexpr int BATCH = sizeof(T) == 4 ? 4 : 2; /* trailing items per prefetch */
+  typedef typename Vec16<T>::type VV;
+  constexpr int VW = Vec16<T>::n;
   extern __shared__ __attribute__((aligned(16))) char smem[];
 
   const int M = a.M;
@@ -180,14 +196,16 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
 
   T *Pn = reinterpret_cast<T *>(smem);
   T *Linv = Pn + (size_t)M * PS;
-  T *prow = Linv + NB * 16; /* [2][NB] */
-  float *redf = reinterpret_cast<float *>(prow + 2 * NB);
+  T *prow = Linv + NB * 16; /* [2][NB] pivot-row broadcast, double buffered */
+  T *dg = prow + 2 * NB;    /* [M] pivots u_kk */
+  double *redd = reinterpret_cast<double *>(dg + M);
+  float *redf = reinterpret_cast<float *>(redd + NW);
+  auto row_of = [](int i) -> int { return i; };
 
   T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * M * M;
 
   for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
     assemble<T>(A, M, a.theta + (size_t)d * a.P, a, tid, NT);
-    double logdet = 0.0;
     int bad = 0;
     T amax = (T)0;
 
@@ -196,15 +214,21 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
       const int nr = M - K;
       const int nq = jb >> 4;
 
-      /* ---- panel rows into registers ---- */
+      /* ---- panel rows into registers (16-byte loads) ---- */
       T v[RPT][NB];
 #pragma unroll
       for (int rr = 0; rr < RPT; ++rr) {
         const int r = tid + NT * rr;
-        const T *src = A + (size_t)(K + r) * M + K;
+        const VV *src = reinterpret_cast<const VV *>(A + (size_t)row_of(K + r < M ? K + r : 0) * M + K);
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-          v[rr][c] = (r < nr && c < jb) ? src[c] : (T)0;
+        for (int c4 = 0; c4 < NB / VW; ++c4) {
+          VV x = {};
+          if (r < nr && c4 * VW < jb)
+            x = src[c4];
+#pragma unroll
+          for (int e = 0; e < VW; ++e)
+            v[rr][c4 * VW + e] = x[e];
+        }
       }
 
       /* ---- factor without interchanges: one barrier per column ---- */
@@ -212,18 +236,23 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
       for (int j = 0; j < NB; ++j) {
         if (j < jb) {
           T *pr = prow + (j & 1) * NB;
-          if (tid == j) {
+          const int c4s = j / VW; /* first 16-byte chunk holding column j */
+            if (tid == j) {
 #pragma unroll
-            for (int c = j; c < NB; ++c)
-              pr[c] = v[0][c];
-          }
-          __syncthreads();
+              for (int c4 = c4s; c4 < NB / VW; ++c4) {
+                VV x;
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                  x[e] = v[0][c4 * VW + e];
+                reinterpret_cast<VV *>(pr)[c4] = x;
+              }
+              dg[K + j] = v[0][j];
+            }
+            __syncthreads();
           T dj = pr[j];
           if (!(fabs(dj) > (T)0)) {
             bad = 1;
             dj = (T)1;
-          } else if (tid == 0) {
-            logdet += log(fabs((double)dj));
           }
 #pragma unroll
           for (int rr = 0; rr < RPT; ++rr) {
@@ -233,8 +262,13 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
               v[rr][j] = l;
               amax = fmax(amax, fabs(l));
 #pragma unroll
-              for (int c = j + 1; c < NB; ++c)
-                v[rr][c] = fma(-l, pr[c], v[rr][c]);
+              for (int c4 = c4s; c4 < NB / VW; ++c4) {
+                const VV x = reinterpret_cast<const VV *>(pr)[c4];
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                  if (c4 * VW + e > j)
+                    v[rr][c4 * VW + e] = fma(-l, x[e], v[rr][c4 * VW + e]);
+              }
             }
           }
         }
@@ -282,7 +316,7 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
             if (q < nq) {
 #pragma unroll
               for (int r = 0; r < 4; ++r)
-                acc[r] = A[(size_t)(K + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+                acc[r] = A[(size_t)row_of(K + 16 * q + rho<T>(g, r)) * M + c0 + l16];
             }
             u[q] = acc;
           }
@@ -303,30 +337,38 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
               u[q] = uq;
 #pragma unroll
               for (int r = 0; r < 4; ++r)
-                A[(size_t)(K + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[q][r];
+                A[(size_t)row_of(K + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[q][r];
             }
           }
         }
         __syncthreads();
 
-        /* ---- phase B: A22 -= L21 U12, balanced (ct, rt) items ---- */
-        const long tot = (long)ntc * ntc;
+        /* ---- phase B: A22 -= L21 U12.  Items are (pair of column tiles =
+         * 32 columns, row tile): full 128-byte rows for FP32, shared A
+         * operands; 8 contiguous item ranges; loads one batch ahead. ---- */
+        const int ntp = (ntc + 1) >> 1;
+        const long tot = (long)ntp * ntc;
         const long lo_it = tot * wave / NW, hi_it = tot * (wave + 1) / NW;
-        int cur_ct = -1;
-        V4 ub[NQ];
-        u16x8 bhi[NKB], blo[NKB];
-        V4 cc[BATCH], cn[BATCH];
-        auto load_batch = [&](long it0, V4 *dst) {
+        int cur_cp = -1;
+        V4 ub[2][NQ];
+        u16x8 bhi[2][NKB], blo[2][NKB];
+        V4 cc[BATCH][2], cn[BATCH][2];
+        auto load_batch = [&](long it0, V4 (*dst)[2]) {
 #pragma unroll
           for (int b = 0; b < BATCH; ++b) {
             const long it = it0 + b;
             if (it < hi_it) {
-              const int ct = (int)(it / ntc);
-              const int rt = (int)(it - (long)ct * ntc);
-              const T *src = A + (size_t)(K + jb + 16 * rt) * M + K + jb + 16 * ct + l16;
+              const int cp = (int)(it / ntc);
+              const int rt = (int)(it - (long)cp * ntc);
+              const int col = K + jb + 32 * cp + l16;
 #pragma unroll
-              for (int r = 0; r < 4; ++r)
-                dst[b][r] = src[(size_t)rho<T>(g, r) * M];
+              for (int r = 0; r < 4; ++r) {
+                const T *src = A + (size_t)row_of(K + jb + 16 * rt + rho<T>(g, r)) * M + col;
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                  if (2 * cp + h < ntc)
+                    dst[b][h][r] = src[16 * h];
+              }
             }
           }
         };
@@ -339,35 +381,39 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
           for (int b = 0; b < BATCH; ++b) {
             const long it = it0 + b;
             if (it < hi_it) {
-              const int ct = (int)(it / ntc);
-              const int rt = (int)(it - (long)ct * ntc);
-              const int c0 = K + jb + 16 * ct;
-              if (ct != cur_ct) {
-                cur_ct = ct;
+              const int cp = (int)(it / ntc);
+              const int rt = (int)(it - (long)cp * ntc);
+              const bool two = 2 * cp + 1 < ntc;
+              if (cp != cur_cp) {
+                cur_cp = cp;
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                  V4 t = {0, 0, 0, 0};
-                  if (q < nq) {
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                      t[r] = A[(size_t)(K + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+                  for (int q = 0; q < NQ; ++q) {
+                    V4 t = {0, 0, 0, 0};
+                    if (q < nq && (h == 0 || two)) {
+#pragma unroll
+                      for (int r = 0; r < 4; ++r)
+                        t[r] = A[(size_t)row_of(K + 16 * q + rho<T>(g, r)) * M + K + jb + 32 * cp + 16 * h + l16];
+                    }
+                    ub[h][q] = t;
                   }
-                  ub[q] = t;
-                }
                 if constexpr (MODE == MODE_BF16X3) {
 #pragma unroll
-                  for (int kb = 0; kb < NKB; ++kb)
+                  for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                      const int q = 2 * kb + (e >> 2);
-                      const float x = (q < NQ) ? (float)ub[q < NQ ? q : 0][e & 3] : 0.f;
-                      const unsigned short h = f2bf(x);
-                      bhi[kb][e] = h;
-                      blo[kb][e] = f2bf(x - bf2f(h));
-                    }
+                    for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+                      for (int e = 0; e < 8; ++e) {
+                        const int q = 2 * kb + (e >> 2);
+                        const float x = (q < NQ) ? (float)ub[h][q < NQ ? q : 0][e & 3] : 0.f;
+                        const unsigned short hh = f2bf(x);
+                        bhi[h][kb][e] = hh;
+                        blo[h][kb][e] = f2bf(x - bf2f(hh));
+                      }
                 }
               }
-              V4 c = cc[b];
+              V4 c0v = cc[b][0], c1v = cc[b][1];
               const int prw = jb + 16 * rt + l16;
               if constexpr (MODE == MODE_BF16X3) {
                 const char *rowp = reinterpret_cast<const char *>(Pn + prw * PS);
@@ -376,9 +422,14 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
                   if (2 * kb < nq) {
                     const u16x8 ahi = *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32);
                     const u16x8 alo = *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32 + 16);
-                    c = mfma_bf16(alo, bhi[kb], c);
-                    c = mfma_bf16(ahi, blo[kb], c);
-                    c = mfma_bf16(ahi, bhi[kb], c);
+                    c0v = mfma_bf16(alo, bhi[0][kb], c0v);
+                    c0v = mfma_bf16(ahi, blo[0][kb], c0v);
+                    c0v = mfma_bf16(ahi, bhi[0][kb], c0v);
+                    if (two) {
+                      c1v = mfma_bf16(alo, bhi[1][kb], c1v);
+                      c1v = mfma_bf16(ahi, blo[1][kb], c1v);
+                      c1v = mfma_bf16(ahi, bhi[1][kb], c1v);
+                    }
                   }
                 }
               } else {
@@ -386,36 +437,56 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
                 for (int aa = 0; aa < NQ; ++aa)
                   if (aa < nq)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                      c = mfma16(-Pn[prw * PS + 16 * aa + rho<T>(g, r)], ub[aa][r], c);
+                    for (int r = 0; r < 4; ++r) {
+                      const T av = -Pn[prw * PS + 16 * aa + rho<T>(g, r)];
+                      c0v = mfma16(av, ub[0][aa][r], c0v);
+                      if (two)
+                        c1v = mfma16(av, ub[1][aa][r], c1v);
+                    }
               }
-              T *dst = A + (size_t)(K + jb + 16 * rt) * M + c0 + l16;
 #pragma unroll
-              for (int r = 0; r < 4; ++r)
-                dst[(size_t)rho<T>(g, r) * M] = (T)c[r];
+              for (int r = 0; r < 4; ++r) {
+                T *dst = A + (size_t)row_of(K + jb + 16 * rt + rho<T>(g, r)) * M + K + jb + 32 * cp + l16;
+                dst[0] = (T)c0v[r];
+                if (two)
+                  dst[16] = (T)c1v[r];
+              }
             }
           }
 #pragma unroll
-          for (int b = 0; b < BATCH; ++b)
-            cc[b] = cn[b];
+          for (int b = 0; b < BATCH; ++b) {
+            cc[b][0] = cn[b][0];
+            cc[b][1] = cn[b][1];
+          }
         }
       }
       __syncthreads();
     }
 
-    /* ---- verify: any |l| > 1 or zero pivot -> partial-pivoting fallback ---- */
+    /* ---- log|det| = sum log|u_kk| (FP64, fixed reduction order) and the
+     * check: any |l| > 1 or zero pivot -> partial-pivoting fallback ---- */
+    double ls = 0.0;
+    for (int i = tid; i < M; i += NT)
+      ls += log(fabs((double)dg[i]));
     float wm = (float)amax;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
+    for (int o = 32; o >= 1; o >>= 1) {
       wm = fmaxf(wm, __shfl_xor(wm, o));
-    if (lane == 0)
+      ls += __shfl_xor(ls, o);
+    }
+    if (lane == 0) {
       redf[wave] = wm;
+      redd[wave] = ls;
+    }
     __syncthreads();
     if (tid == 0) {
       float mx = redf[0];
-      for (int w = 1; w < NW; ++w)
+      double logdet = redd[0];
+      for (int w = 1; w < NW; ++w) {
         mx = fmaxf(mx, redf[w]);
-      if (bad || !(mx <= 1.0f)) {
+        logdet += redd[w];
+      }
+      if (bad || !(mx <= kMaxMultiplier)) {
         const int slot = atomicAdd(a.flag_count, 1);
         a.flag_list[slot] = (int)d;
         a.logdet[d] = NAN;
@@ -701,9 +772,10 @@ __global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a) {
 }
 
 size_t fast_lds_bytes(int M, int NB, int MODE, size_t ts) {
-  size_t b = (size_t)M * panel_stride(NB, MODE) * ts + (size_t)NB * 16 * ts + 2 * (size_t)NB * ts;
+  size_t b = (size_t)M * panel_stride(NB, MODE) * ts + (size_t)NB * 16 * ts + 2 * (size_t)NB * ts +
+             (size_t)M * ts;
   b = (b + 15) & ~(size_t)15;
-  return b + 16 * sizeof(float);
+  return b + 8 * sizeof(double) + 8 * sizeof(float);
 }
 
 size_t pivot_lds_bytes(int M, int NB, size_t ts, bool solve) {
@@ -729,8 +801,13 @@ template <typename T, int NB, int MODE>
 int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   if (a.M <= 512)
     return launch_kernel(k_lu_fast<T, NB, MODE, 1>, plan.lds, plan.grid, 512, a, st);
-  if (a.M <= 1024)
-    return launch_kernel(k_lu_fast<T, NB, MODE, 2>, plan.lds, plan.grid, 512, a, st);
+  /* FP64 with 16-wide panels and two rows per thread is never planned (see
+   * lu_plan); not instantiating it also avoids a register-coalescer crash of
+   * the ROCm 7.2 compiler on that instantiation. */
+  if constexpr (!(sizeof(T) == 8 && NB == 16)) {
+    if (a.M <= 1024)
+      return launch_kernel(k_lu_fast<T, NB, MODE, 2>, plan.lds, plan.grid, 512, a, st);
+  }
   return SGDLM_ERR_ARG;
 }
 
@@ -802,7 +879,8 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     if (prec == SGDLM_PREC_BF16X3 && nb < 32)
       continue;
     const size_t lds = fast_lds_bytes(M, nb, prec == SGDLM_PREC_BF16X3 ? MODE_BF16X3 : MODE_NATIVE, ts);
-    if (lds <= lds_max && M <= 1024) {
+    const bool fp64_narrow_wide = prec == SGDLM_PREC_FP64 && nb == 16 && M > 512;
+    if (lds <= lds_max && M <= 1024 && !fp64_narrow_wide) {
       plan->nb = nb;
       plan->lds = lds;
     }
@@ -817,7 +895,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
       plan->pivot_lds = lds;
     }
   }
-  if (plan->pivot_nb == 0 || (!solve && plan->nb == 0))
+  if (plan->pivot_nb == 0)
     return SGDLM_ERR_ARG;
   plan->grid = n_cu > 0 ? n_cu : 256;
   plan->work_bytes = (size_t)plan->grid * M * M * ts;
@@ -831,6 +909,14 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   a.logdet = logdet;
   a.flag_count = flags;
   a.flag_list = flags + 1;
+  if (plan.nb == 0) {
+    /* no fast configuration: partial pivoting for every draw */
+    a.flag_count = nullptr;
+    a.flag_list = nullptr;
+    if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
+      return SGDLM_ERR_DEVICE;
+    return dispatch_pivot<false>(a, plan, st);
+  }
   if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
     return SGDLM_ERR_DEVICE;
   int rc = dispatch_fast(a, plan, st);

@@ -160,9 +160,10 @@ def test_forecast_fp32_refined_m100(hiplib):
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 2e-3)])
 def test_logdet_pivoting_fallback(hiplib, prec, tol):
-    """Posterior means of the simultaneous coefficients near 1.5 force row
-    interchanges (|l| > 1) in most draws: those draws go through the
-    partial-pivoting kernel and must still match the oracle's LU."""
+    """Posterior means of the simultaneous coefficients near +-2.5 force
+    multipliers |l| > 2 in most draws: those draws go through the
+    partial-pivoting kernel (the fallback count says so) and must still match
+    the oracle's LU."""
     mod = readme_model(T=30)
     g, o = make_pair(mod, prec)
     filter_steps(g, mod, 1, 10)
@@ -175,12 +176,13 @@ def test_logdet_pivoting_fallback(hiplib, prec, tol):
     for j in range(mod.m):
         for k in range(mod.max_p):
             if spf[k, j] < mod.m * mod.m:
-                mt[k, j] = 1.5 if (j + k) % 2 else -1.3
+                mt[k, j] = 2.5 if (j + k) % 2 else -2.3
     for api in (g, o):
         api.setPosteriorParameters(mt, prm["C"] * 0.01, prm["n"], prm["s"])
     N = 2000
     g.computeVBPosterior(N, N)
     lw = g.get_vb_result()["log_weights"]
+    assert g.lu_fallback_count() >= N // 4
     lam, th = orc.sample(mod.m, mod.max_p, mod.p, mt, prm["C"] * 0.01, prm["n"], prm["s"], N, SEED, 1)
     ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
     fin = np.isfinite(ld)
