@@ -102,7 +102,19 @@ def main():
     ap.add_argument("--filter-steps", type=int, default=250)
     ap.add_argument("--precisions", default="fp32,fp64,bf16x3")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB MiB (counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": a.copy_calibration, "kernels": 3}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this engine has no multi-GPU path "
               "(draw sharding is a design note, not built)", file=sys.stderr)
