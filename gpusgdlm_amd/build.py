@@ -2,6 +2,8 @@
 
   gpusgdlm_amd/lib/libsgdlm_hip.so   -- HIP kernels + C-ABI (include/sgdlm.h)
   gpusgdlm_amd/lib/libsgdlm_host.so  -- C++ SGDLM::HostWrapper<double> over it
+  gpusgdlm_amd/lib/host_wrapper_driver -- test program of the C++ interface
+                                          (tests/cpp/host_wrapper_driver.cpp)
 
 Usage: python -m gpusgdlm_amd.build [--jobs N]
 """
@@ -70,6 +72,12 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
         _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-I", os.path.join(ROOT, "include"),
               *srcs, "-o", host + ".tmp", "-L", LIB, "-lsgdlm_hip", "-Wl,-rpath,$ORIGIN"])
         os.replace(host + ".tmp", host)
+        drv_src = os.path.join(ROOT, "tests", "cpp", "host_wrapper_driver.cpp")
+        if os.path.exists(drv_src):
+            drv = os.path.join(LIB, "host_wrapper_driver")
+            _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), drv_src,
+                  "-o", drv + ".tmp", "-L", LIB, "-lsgdlm_host", "-Wl,-rpath,$ORIGIN"])
+            os.replace(drv + ".tmp", drv)
     if verbose:
         print(f"built {so}")
     return so

@@ -5,7 +5,7 @@
  * SGDLM::HostWrapper<double> (rSGDLM/src/SGDLM/HostWrapper.hpp:13-99), which
  * the Rcpp module gpuSGDLM/SGDLM calls (rSGDLM/src/SGDLM/RcppWrapper.cpp).
  * A C++ implementation of that interface over these functions is in
- * include/sgdlm/HostWrapper.hpp + gpusgdlm_amd/csrc/host_wrapper.cpp.
+ * include/SGDLM/HostWrapper.hpp + gpusgdlm_amd/csrc/host_wrapper.cpp.
  *
  * Conventions
  *   - Every function returns an int status (SGDLM_OK on success).  The
