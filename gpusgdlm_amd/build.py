@@ -4,6 +4,8 @@
   gpusgdlm_amd/lib/libsgdlm_host.so  -- C++ SGDLM::HostWrapper<double> over it
   gpusgdlm_amd/lib/host_wrapper_driver -- test program of the C++ interface
                                           (tests/cpp/host_wrapper_driver.cpp)
+  gpusgdlm_amd/lib/libelim_check.so    -- CPU execution of the LU elimination
+                                          plan (tests/cpp/elim_check.cpp)
 
 Usage: python -m gpusgdlm_amd.build [--jobs N]
 """
@@ -23,7 +25,7 @@ LIB = os.path.join(HERE, "lib")
 OBJ = os.path.join(LIB, "obj")
 
 ARCH = "gfx950"
-HIP_SOURCES = ["k_filter.hip", "k_draws.hip", "k_lu.hip", "k_vb.hip", "sgdlm_capi.hip"]
+HIP_SOURCES = ["k_filter.hip", "k_draws.hip", "k_lu.hip", "k_vb.hip", "sgdlm_capi.hip", "lu_symbolic.cpp"]
 HOST_SOURCES = ["host_wrapper.cpp"]
 
 HIPCC_FLAGS = [
@@ -58,7 +60,7 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     objs = []
     cmds = []
     for src in HIP_SOURCES:
-        o = os.path.join(OBJ, src.replace(".hip", ".o"))
+        o = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         objs.append(o)
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -78,6 +80,12 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
             _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), drv_src,
                   "-o", drv + ".tmp", "-L", LIB, "-lsgdlm_host", "-Wl,-rpath,$ORIGIN"])
             os.replace(drv + ".tmp", drv)
+    chk_src = os.path.join(ROOT, "tests", "cpp", "elim_check.cpp")
+    if os.path.exists(chk_src):
+        chk = os.path.join(LIB, "libelim_check.so")
+        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", chk_src,
+              os.path.join(CSRC, "lu_symbolic.cpp"), "-o", chk + ".tmp"])
+        os.replace(chk + ".tmp", chk)
     if verbose:
         print(f"built {so}")
     return so

@@ -14,6 +14,13 @@
  * Infinity Cache, and is assembled there from the packed coefficient draws.
  *
  * k_lu_fast (VB log-determinants; 8 wavefronts):
+ *   Sparse prefix (tables from lu_symbolic.cpp): the low-degree series of
+ *   the parental graph are eliminated first, wave by wave, on LDS slots
+ *   holding only the entries their elimination touches; the Schur complement
+ *   on the remaining separator series is gathered into the dense workspace
+ *   (M = separator count rounded up to 16) and factored by the blocked LU
+ *   below.  At m = 400 with up to 20 parents per series this removes a third
+ *   of the series from the dense LU (about 3x fewer flops).
  *   Threshold partial pivoting keeps the diagonal pivot while every
  *   multiplier satisfies |l_ij| <= 1/tau (LAPACK's partial pivoting is
  *   tau = 1 and then performs no interchange exactly when all |l_ij| <= 1).
@@ -124,6 +131,14 @@ struct LuArgs {
   int refine;
   int *flag_count; /* fast: appends flagged draws; pivot: number to process */
   int *flag_list;
+  /* sparse prefix (fast kernel) */
+  int e_n1, e_ns, e_waves, e_nsg, e_nblk, e_tab_words;
+  int e_o_piv, e_o_mptr, e_o_mult, e_o_tptr, e_o_tslot, e_o_cptr, e_o_contrib, e_o_brounds,
+      e_o_bcbase;
+  unsigned e_zero_pair;
+  int e_pre_base, e_slot_bytes, e_tab_persist;
+  const int *e_slot_init, *e_tab, *e_sg_off, *e_sg_idx, *e_st_off;
+  const unsigned *e_st_pairs;
 };
 
 template <typename T>
@@ -139,6 +154,118 @@ __device__ void assemble(T *A, int M, const double *th, const LuArgs &a, int tid
     A[(size_t)r * M + r] = (T)1;
   for (int e = tid; e < a.n_gam; e += nthreads)
     A[(size_t)a.g_row[e] * M + a.g_col[e]] = (T)(-th[a.g_idx[e]]);
+  __syncthreads();
+}
+
+/* Sparse prefix + Schur-complement assembly of one draw (fast kernel).
+ * LDS at a.e_pre_base: slots sl[] (the entries the prefix elimination
+ * touches, plus constant and base slots), then the prefix tables tb[]
+ * (copied from a.e_tab on every draw unless they persist).  Per wave the
+ * multipliers l_ik = a_ik / a_kk replace a_ik, then every target slot
+ * subtracts its gathered sum of l_ik * a_kj (fixed order).  The dense
+ * workspace S (M x M, identity padding) receives the separator block of
+ * I - Gamma; every entry the prefix updates is then overwritten with
+ * -sum x * y over its slot pairs, read in ELL blocks of 64 (one per lane),
+ * several blocks in flight per wavefront.  Accumulates the prefix pivots'
+ * log|a_kk| (FP64), the zero-pivot flag and max |l|. */
+template <typename T>
+__device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const LuArgs &a,
+                                int tid, int nthreads, bool load_tab, int &bad, T &amax,
+                                double &ls) {
+  constexpr unsigned LO = 0xffffu;
+  constexpr int NWV = 8;  /* wavefronts of the fast kernel */
+  constexpr int UB = 4;   /* Schur blocks in flight per wavefront */
+  constexpr int RR = 8;   /* rounds loaded ahead per block */
+  const int lane = tid & 63, wave = tid >> 6;
+  T *sl = reinterpret_cast<T *>(smem + a.e_pre_base);
+  int *tb = reinterpret_cast<int *>(smem + a.e_pre_base + a.e_slot_bytes);
+  if (load_tab) {
+    for (int w = tid; w < a.e_tab_words; w += nthreads)
+      tb[w] = a.e_tab[w];
+  }
+  for (int s = tid; s < a.e_ns; s += nthreads) {
+    const int v = a.e_slot_init[s];
+    sl[s] = v >= 0 ? (T)(-th[v]) : (v == -2 ? (T)1 : (v == -3 ? (T)-1 : (T)0));
+  }
+  const size_t nvec = (size_t)M * M * sizeof(T) / 16;
+  float4 *A4 = reinterpret_cast<float4 *>(A);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (size_t q = tid; q < nvec; q += nthreads)
+    A4[q] = z;
+  __syncthreads();
+  const int *mptr = tb + a.e_o_mptr, *tptr = tb + a.e_o_tptr, *cptr = tb + a.e_o_cptr;
+  const unsigned *mult = reinterpret_cast<const unsigned *>(tb + a.e_o_mult);
+  const unsigned *contrib = reinterpret_cast<const unsigned *>(tb + a.e_o_contrib);
+  const int *tslot = tb + a.e_o_tslot;
+  for (int w = 0; w < a.e_waves; ++w) {
+    const int m1 = mptr[w + 1];
+    for (int it = mptr[w] + tid; it < m1; it += nthreads) {
+      const unsigned x = mult[it];
+      T d = sl[x >> 16];
+      if (!(fabs(d) > (T)0)) {
+        bad = 1;
+        d = (T)1;
+      }
+      const T l = sl[x & LO] / d;
+      sl[x & LO] = l;
+      amax = fmax(amax, fabs(l));
+    }
+    __syncthreads();
+    const int t1 = tptr[w + 1];
+    for (int t = tptr[w] + tid; t < t1; t += nthreads) {
+      const int s = tslot[t];
+      T acc = sl[s];
+      const int e1 = cptr[t + 1];
+      for (int e = cptr[t]; e < e1; ++e) {
+        const unsigned c = contrib[e];
+        acc = fma(-sl[c & LO], sl[c >> 16], acc);
+      }
+      sl[s] = acc;
+    }
+    __syncthreads();
+  }
+  const int *piv = tb + a.e_o_piv;
+  for (int q = tid; q < a.e_n1; q += nthreads) {
+    const T p = sl[piv[q]];
+    if (!(fabs(p) > (T)0))
+      bad = 1;
+    ls += log(fabs((double)p));
+  }
+  for (int r = tid; r < M; r += nthreads)
+    A[(size_t)r * M + r] = (T)1;
+  for (int e = tid; e < a.e_nsg; e += nthreads)
+    A[a.e_sg_off[e]] = (T)(-th[a.e_sg_idx[e]]);
+  __syncthreads();
+  const int *brounds = tb + a.e_o_brounds, *bcbase = tb + a.e_o_bcbase;
+  const unsigned *pairs = a.e_st_pairs;
+  for (int b0 = wave; b0 < a.e_nblk; b0 += NWV * UB) {
+    int off[UB], R[UB], cb[UB];
+    unsigned pv[UB][RR];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int b = b0 + NWV * u;
+      const bool ok = b < a.e_nblk;
+      R[u] = ok ? brounds[b] : 0;
+      cb[u] = ok ? bcbase[b] : 0;
+      off[u] = ok ? a.e_st_off[(size_t)b * 64 + lane] : -1;
+#pragma unroll
+      for (int r = 0; r < RR; ++r)
+        pv[u][r] = r < R[u] ? pairs[(size_t)cb[u] + 64 * r + lane] : a.e_zero_pair;
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      T acc = (T)0;
+#pragma unroll
+      for (int r = 0; r < RR; ++r)
+        acc = fma(-sl[pv[u][r] & LO], sl[pv[u][r] >> 16], acc);
+      for (int r = RR; r < R[u]; ++r) {
+        const unsigned c = pairs[(size_t)cb[u] + 64 * r + lane];
+        acc = fma(-sl[c & LO], sl[c >> 16], acc);
+      }
+      if (off[u] >= 0)
+        A[off[u]] = acc;
+    }
+  }
   __syncthreads();
 }
 
@@ -205,9 +332,11 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
   T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * M * M;
 
   for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
-    assemble<T>(A, M, a.theta + (size_t)d * a.P, a, tid, NT);
     int bad = 0;
     T amax = (T)0;
+    double ls = 0.0;
+    prefix_assemble<T>(A, M, smem, a.theta + (size_t)d * a.P, a, tid, NT,
+                       !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls);
 
     for (int K = 0; K < M; K += NB) {
       const int jb = min(NB, M - K);
@@ -237,18 +366,18 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
         if (j < jb) {
           T *pr = prow + (j & 1) * NB;
           const int c4s = j / VW; /* first 16-byte chunk holding column j */
-            if (tid == j) {
+          if (tid == j) {
 #pragma unroll
-              for (int c4 = c4s; c4 < NB / VW; ++c4) {
-                VV x;
+            for (int c4 = c4s; c4 < NB / VW; ++c4) {
+              VV x;
 #pragma unroll
-                for (int e = 0; e < VW; ++e)
-                  x[e] = v[0][c4 * VW + e];
-                reinterpret_cast<VV *>(pr)[c4] = x;
-              }
-              dg[K + j] = v[0][j];
+              for (int e = 0; e < VW; ++e)
+                x[e] = v[0][c4 * VW + e];
+              reinterpret_cast<VV *>(pr)[c4] = x;
             }
-            __syncthreads();
+            dg[K + j] = v[0][j];
+          }
+          __syncthreads();
           T dj = pr[j];
           if (!(fabs(dj) > (T)0)) {
             bad = 1;
@@ -465,10 +594,10 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
 
     /* ---- log|det| = sum log|u_kk| (FP64, fixed reduction order) and the
      * check: any |l| > 1 or zero pivot -> partial-pivoting fallback ---- */
-    double ls = 0.0;
     for (int i = tid; i < M; i += NT)
       ls += log(fabs((double)dg[i]));
-    float wm = (float)amax;
+    /* a zero pivot seen by any thread flags the draw through the max */
+    float wm = bad ? INFINITY : (float)amax;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
       wm = fmaxf(wm, __shfl_xor(wm, o));
@@ -486,7 +615,7 @@ __global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
         mx = fmaxf(mx, redf[w]);
         logdet += redd[w];
       }
-      if (bad || !(mx <= kMaxMultiplier)) {
+      if (!(mx <= kMaxMultiplier)) {
         const int slot = atomicAdd(a.flag_count, 1);
         a.flag_list[slot] = (int)d;
         a.logdet[d] = NAN;
@@ -859,6 +988,28 @@ LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
   a.g_idx = S.g_idx;
   a.g_rowptr = S.g_rowptr;
   a.work = work;
+  a.e_n1 = S.e_n1;
+  a.e_ns = S.e_ns;
+  a.e_waves = S.e_waves;
+  a.e_nsg = S.e_nsg;
+  a.e_nblk = S.e_nblk;
+  a.e_tab_words = S.e_tab_words;
+  a.e_o_piv = S.e_o_piv;
+  a.e_o_mptr = S.e_o_mptr;
+  a.e_o_mult = S.e_o_mult;
+  a.e_o_tptr = S.e_o_tptr;
+  a.e_o_tslot = S.e_o_tslot;
+  a.e_o_cptr = S.e_o_cptr;
+  a.e_o_contrib = S.e_o_contrib;
+  a.e_o_brounds = S.e_o_brounds;
+  a.e_o_bcbase = S.e_o_bcbase;
+  a.e_zero_pair = S.e_zero_pair;
+  a.e_slot_init = S.e_slot_init;
+  a.e_tab = S.e_tab;
+  a.e_sg_off = S.e_sg_off;
+  a.e_sg_idx = S.e_sg_idx;
+  a.e_st_off = S.e_st_off;
+  a.e_st_pairs = S.e_st_pairs;
   return a;
 }
 
@@ -867,22 +1018,37 @@ LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             LuPlan *plan) {
   const int M = S.M;
+  const int M2 = S.e_M2;
   const size_t ts = (prec == SGDLM_PREC_FP64) ? 8 : 4;
   const int cands[3] = {64, 32, 16};
   plan->prec = prec;
   plan->nb = 0;
   plan->pivot_nb = 0;
-  for (int i = 0; i < 3 && !plan->nb; ++i) {
+  plan->fast_M = M2;
+  const size_t slot_bytes = ((size_t)S.e_ns * ts + 15) & ~(size_t)15;
+  const size_t pre_bytes = slot_bytes + (size_t)S.e_tab_words * 4;
+  for (int i = 0; i < 3 && !plan->nb && !solve; ++i) {
     const int nb = cands[i];
     if (prec == SGDLM_PREC_FP64 && nb > 32)
       continue;
     if (prec == SGDLM_PREC_BF16X3 && nb < 32)
       continue;
-    const size_t lds = fast_lds_bytes(M, nb, prec == SGDLM_PREC_BF16X3 ? MODE_BF16X3 : MODE_NATIVE, ts);
-    const bool fp64_narrow_wide = prec == SGDLM_PREC_FP64 && nb == 16 && M > 512;
-    if (lds <= lds_max && M <= 1024 && !fp64_narrow_wide) {
+    const size_t dense =
+        fast_lds_bytes(M2, nb, prec == SGDLM_PREC_BF16X3 ? MODE_BF16X3 : MODE_NATIVE, ts);
+    const bool fp64_narrow_wide = prec == SGDLM_PREC_FP64 && nb == 16 && M2 > 512;
+    if (M2 > 1024 || fp64_narrow_wide)
+      continue;
+    if (dense + pre_bytes <= lds_max) {
+      /* prefix region after the dense one: tables loaded once per workgroup */
       plan->nb = nb;
-      plan->lds = lds;
+      plan->lds = dense + pre_bytes;
+      plan->pre_base = dense;
+      plan->tab_persist = 1;
+    } else if (std::max(dense, pre_bytes) <= lds_max) {
+      plan->nb = nb;
+      plan->lds = std::max(dense, pre_bytes);
+      plan->pre_base = 0;
+      plan->tab_persist = 0;
     }
   }
   for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
@@ -919,9 +1085,14 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   }
   if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
     return SGDLM_ERR_DEVICE;
+  a.M = plan.fast_M;
+  a.e_pre_base = (int)plan.pre_base;
+  a.e_tab_persist = plan.tab_persist;
+  a.e_slot_bytes = (int)((((size_t)S.e_ns * (plan.prec == SGDLM_PREC_FP64 ? 8 : 4)) + 15) & ~(size_t)15);
   int rc = dispatch_fast(a, plan, st);
   if (rc)
     return rc;
+  a.M = S.M;
   return dispatch_pivot<false>(a, plan, st);
 }
 

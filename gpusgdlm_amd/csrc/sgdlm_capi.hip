@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/sgdlm.h"
+#include "lu_symbolic.h"
 #include "sgdlm_internal.h"
 
 using namespace sgdlm;
@@ -61,6 +62,7 @@ struct sgdlm_ctx {
   size_t nsim = 0, nsim_batch = 0;
 
   int prec = SGDLM_PREC_FP64;
+  int max_front = 64; /* sparse LU prefix: largest pivot degree (0 = off) */
   uint64_t seed = 1234ull;
   uint32_t epoch = 0;
 
@@ -265,6 +267,63 @@ int d2h(sgdlm_ctx *c, void *dst, const void *src, size_t bytes) {
   return SGDLM_OK;
 }
 
+template <typename V> int upload(sgdlm_ctx *c, const char *name, const V &v) {
+  const size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(typename V::value_type);
+  int rc;
+  if ((rc = dalloc(c, c->mem, name, bytes)))
+    return rc;
+  if (!v.empty() && (rc = h2d(c, c->mem[name].ptr, v.data(), v.size() * sizeof(v[0]))))
+    return rc;
+  return SGDLM_OK;
+}
+
+/* Symbolic elimination plan of the VB log-determinant LU for the current
+ * parental sets (lu_symbolic.h).  Its LDS slots (FP64) and tables must fit
+ * the LDS of one workgroup; the front limit is halved until they do. */
+int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
+  ElimPlan P;
+  const size_t budget = c->lds_max > 1024 ? c->lds_max - 1024 : 0;
+  for (int mf = c->max_front;; mf /= 2) {
+    build_elim_plan((int)c->m, ents, mf, (int)std::min<size_t>(65536, budget / 8), P);
+    const size_t need = (size_t)P.ns * 8 + 16 + P.tab.size() * 4;
+    if (need <= budget || mf == 0)
+      break;
+  }
+  int rc;
+  if ((rc = upload(c, "e_slot_init", P.slot_init)) || (rc = upload(c, "e_tab", P.tab)) ||
+      (rc = upload(c, "e_sg_off", P.sg_off)) || (rc = upload(c, "e_sg_idx", P.sg_idx)) ||
+      (rc = upload(c, "e_st_off", P.st_off)) || (rc = upload(c, "e_st_pairs", P.st_pairs)))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Structure &S = c->S;
+  S.e_n1 = P.n1;
+  S.e_n2 = P.n2;
+  S.e_M2 = P.M2;
+  S.e_ns = P.ns;
+  S.e_waves = P.n_waves;
+  S.e_nsg = (int)P.sg_off.size();
+  S.e_nblk = P.n_blk;
+  S.e_tab_words = (int)P.tab.size();
+  S.e_o_piv = P.o_piv;
+  S.e_o_mptr = P.o_mptr;
+  S.e_o_mult = P.o_mult;
+  S.e_o_tptr = P.o_tptr;
+  S.e_o_tslot = P.o_tslot;
+  S.e_o_cptr = P.o_cptr;
+  S.e_o_contrib = P.o_contrib;
+  S.e_o_brounds = P.o_brounds;
+  S.e_o_bcbase = P.o_bcbase;
+  S.e_zero_pair = P.zero_pair;
+  S.e_fma = P.prefix_fma;
+  S.e_slot_init = M_<int>(c, "e_slot_init");
+  S.e_tab = M_<int>(c, "e_tab");
+  S.e_sg_off = M_<int>(c, "e_sg_off");
+  S.e_sg_idx = M_<int>(c, "e_sg_idx");
+  S.e_st_off = M_<int>(c, "e_st_off");
+  S.e_st_pairs = M_<uint32_t>(c, "e_st_pairs");
+  return SGDLM_OK;
+}
+
 /* Rebuild the device-side parent structure from h_p / h_sp. */
 int build_structure(sgdlm_ctx *c) {
   const int m = (int)c->m, mp = (int)c->max_p;
@@ -272,9 +331,7 @@ int build_structure(sgdlm_ctx *c) {
   std::vector<int> off(m + 1, 0);
   for (int j = 0; j < m; ++j)
     off[j + 1] = off[j] + (int)c->h_p[j];
-  struct E {
-    int row, col, idx;
-  };
+  typedef GammaEntry E;
   std::vector<E> ents;
   for (int j = 0; j < m; ++j)
     for (int k = 0; k < (int)c->h_p[j]; ++k) {
@@ -323,7 +380,7 @@ int build_structure(sgdlm_ctx *c) {
   S.g_col = M_<int>(c, "g_col");
   S.g_idx = M_<int>(c, "g_idx");
   S.g_rowptr = M_<int>(c, "g_rowptr");
-  return SGDLM_OK;
+  return build_elimination(c, ents);
 }
 
 int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
@@ -1034,6 +1091,37 @@ int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb) {
     *lds = plan.lds;
   if (nb)
     *nb = plan.nb;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_lu_max_front(sgdlm_ctx *c, int max_front) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (max_front < 0)
+    return fail(c, SGDLM_ERR_ARG, "max_front must be >= 0");
+  c->max_front = max_front;
+  c->vb_valid = false;
+  if (c->parents_set)
+    return build_structure(c);
+  return SGDLM_OK;
+}
+
+int sgdlm_get_lu_elimination(const sgdlm_ctx *c, int *max_front, int *n_prefix, int *n_dense,
+                             int *n_waves, long *prefix_fma) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (max_front)
+    *max_front = c->max_front;
+  if (!c->parents_set)
+    return SGDLM_ERR_STATE;
+  if (n_prefix)
+    *n_prefix = c->S.e_n1;
+  if (n_dense)
+    *n_dense = c->S.e_n2;
+  if (n_waves)
+    *n_waves = c->S.e_waves;
+  if (prefix_fma)
+    *prefix_fma = (long)c->S.e_fma;
   return SGDLM_OK;
 }
 

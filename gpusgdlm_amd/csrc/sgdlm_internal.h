@@ -26,6 +26,17 @@ struct Structure {
   /* Gamma entries sorted by (row, col): A[row][col] = -theta_packed[idx]   */
   const int *g_row = nullptr, *g_col = nullptr, *g_idx = nullptr;
   const int *g_rowptr = nullptr; /* device, m + 1 (CSR over rows)           */
+  /* Sparse elimination prefix of the log-determinant LU (lu_symbolic.h):
+   * host sizes and device tables. */
+  int e_n1 = 0, e_n2 = 0, e_M2 = 0, e_ns = 0, e_waves = 0, e_nsg = 0, e_nblk = 0;
+  int e_tab_words = 0;
+  int e_o_piv = 0, e_o_mptr = 0, e_o_mult = 0, e_o_tptr = 0, e_o_tslot = 0, e_o_cptr = 0,
+      e_o_contrib = 0, e_o_brounds = 0, e_o_bcbase = 0;
+  uint32_t e_zero_pair = 0;
+  long long e_fma = 0;
+  const int *e_slot_init = nullptr, *e_tab = nullptr;
+  const int *e_sg_off = nullptr, *e_sg_idx = nullptr, *e_st_off = nullptr;
+  const uint32_t *e_st_pairs = nullptr;
 };
 
 /* ---- filter (k_filter.hip) ---- */
@@ -60,9 +71,13 @@ void launch_unpack_theta(const Structure &S, long N, const double *theta,
 struct LuPlan {
   int prec = SGDLM_PREC_FP64; /* element type / update path                */
   int nb = 0;                 /* panel width, no-interchange fast kernel    */
+  int fast_M = 0;             /* dense dimension of the fast kernel (the
+                                 Schur complement after the sparse prefix)  */
   int pivot_nb = 0;           /* panel width, partial-pivoting kernel       */
   int grid = 0;               /* persistent workgroups                      */
   size_t lds = 0;             /* dynamic LDS bytes, fast kernel             */
+  size_t pre_base = 0;        /* LDS byte offset of the prefix slots/tables */
+  int tab_persist = 0;        /* prefix tables stay in LDS across draws     */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
   size_t work_bytes = 0;      /* global workspace bytes (grid * M * M * T)  */
 };

@@ -351,3 +351,15 @@ class SGDLM:
         g, l, nb = C.c_int(), C.c_size_t(), C.c_int()
         self._call("sgdlm_get_lu_config", C.byref(g), C.byref(l), C.byref(nb))
         return {"grid": g.value, "lds": l.value, "nb": nb.value}
+
+    def set_lu_max_front(self, max_front: int):
+        """Largest pivot degree of the sparse LU prefix (0 = dense LU only)."""
+        self._call("sgdlm_set_lu_max_front", C.c_int(int(max_front)))
+
+    def lu_elimination(self):
+        mf, n1, n2, nw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        fma = C.c_long()
+        self._call("sgdlm_get_lu_elimination", C.byref(mf), C.byref(n1), C.byref(n2), C.byref(nw),
+                   C.byref(fma))
+        return {"max_front": mf.value, "n_prefix": n1.value, "n_dense": n2.value, "waves": nw.value,
+                "prefix_fma": fma.value}

@@ -158,6 +158,17 @@ int sgdlm_get_lu_fallback_count(const sgdlm_ctx *ctx, long *count);
 /* Number of workgroups / LDS bytes / panel width chosen for the VB LU. */
 int sgdlm_get_lu_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
                         int *nb);
+/* Sparse prefix of the VB log-determinant LU: series whose degree in the
+ * (symmetrised) parental graph stays <= max_front are eliminated by a
+ * minimum-degree schedule before the dense LU of the remaining series
+ * (default 64; 0 = dense LU of the whole I - Gamma).  The result is the
+ * same determinant; only the rounding differs. */
+int sgdlm_set_lu_max_front(sgdlm_ctx *ctx, int max_front);
+/* The plan for the current parental sets: series eliminated in the prefix,
+ * series in the dense LU, prefix waves and prefix multiply-adds per draw. */
+int sgdlm_get_lu_elimination(const sgdlm_ctx *ctx, int *max_front,
+                             int *n_prefix, int *n_dense, int *n_waves,
+                             long *prefix_fma);
 
 #ifdef __cplusplus
 }

@@ -199,3 +199,56 @@ def test_logdet_pivoting_fallback(hiplib, prec, tol):
                 A[v % mod.m, v // mod.m] = -thf[q]
         swaps += int(np.any(sl.lu_factor(A)[1] != np.arange(mod.m)))
     assert swaps >= 5
+
+
+def _logdets(mod, prec, N, max_front, t_post=8):
+    g, o = make_pair(mod, prec)
+    g.set_lu_max_front(max_front)
+    filter_steps(g, mod, 1, t_post)
+    g.computePosterior(mod.y[t_post], mod.F(t_post))
+    g.computeVBPosterior(N, N)
+    lw = g.get_vb_result()["log_weights"]
+    el = g.lu_elimination()
+    fb = g.lu_fallback_count()
+    g.close()
+    filter_steps(o, mod, 1, t_post)
+    o.computePosterior(mod.y[t_post], mod.F(t_post))
+    return lw, el, fb, o
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-3), ("bf16x3", 5e-3)])
+def test_logdet_sparse_prefix_m400(hiplib, prec, tol):
+    """m = 400, k ~ U{0..20}: the minimum-degree prefix eliminates about a
+    third of the series before the dense LU; log-dets match the oracle's
+    partial-pivoting LU (FP64: 1e-10 absolute) and the dense-only path."""
+    mod = random_model(400, 20, T=45, seed=7)
+    N = 128
+    print(f"sparse prefix m=400 {prec} N={N}", flush=True)
+    lw, el, fb, o = _logdets(mod, prec, N, 64, t_post=40)
+    lw0, el0, _, _ = _logdets(mod, prec, N, 0, t_post=40)
+    assert el["n_prefix"] > 100 and el["n_dense"] + el["n_prefix"] == 400 and el["waves"] >= 1
+    assert el0["n_prefix"] == 0 and el0["n_dense"] == 400
+    prm = o.getParameters()
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
+    print(f"  {el}: max |err| prefix {np.max(np.abs(lw - ld)):.3e} dense {np.max(np.abs(lw0 - ld)):.3e}")
+    assert np.max(np.abs(lw - ld)) <= tol
+    assert np.max(np.abs(lw0 - ld)) <= tol
+    assert fb <= N // 8
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 5e-3)])
+def test_logdet_sparse_prefix_m1000(hiplib, prec, tol):
+    """m = 1000, k ~ U{0..30}: dense Schur complement of ~750 series (two
+    panel rows per thread)."""
+    mod = random_model(1000, 30, T=205, seed=5)
+    N = 32
+    print(f"sparse prefix m=1000 {prec} N={N}", flush=True)
+    lw, el, fb, o = _logdets(mod, prec, N, 64, t_post=200)
+    assert 600 < el["n_dense"] < 900
+    prm = o.getParameters()
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
+    print(f"  {el}: max |err| {np.max(np.abs(lw - ld)):.3e}, fallback {fb}")
+    assert np.max(np.abs(lw - ld)) <= tol
+    assert fb <= N // 8
