@@ -66,7 +66,9 @@ SIGNATURES = {
     "sgdlm_set_timing": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_get_timing": (C.c_int, [_ctx, C.c_char_p, _dp, C.POINTER(C.c_long)]),
     "sgdlm_get_lu_fallback_count": (C.c_int, [_ctx, C.POINTER(C.c_long)]),
-    "sgdlm_get_lu_config": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
+    "sgdlm_get_lu_config": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
+    "sgdlm_set_lu_tuning": (C.c_int, [_ctx, C.c_int, C.c_int]),
     "sgdlm_set_lu_max_front": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_get_lu_elimination": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.POINTER(C.c_long)]),

@@ -173,7 +173,7 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
                                 int tid, int nthreads, bool load_tab, int &bad, T &amax,
                                 double &ls) {
   constexpr unsigned LO = 0xffffu;
-  constexpr int NWV = 8;  /* wavefronts of the fast kernel */
+  const int NWV = nthreads >> 6; /* wavefronts of the workgroup */
   constexpr int UB = 4;   /* Schur blocks in flight per wavefront */
   constexpr int RR = 8;   /* rounds loaded ahead per block */
   const int lane = tid & 63, wave = tid >> 6;
@@ -301,10 +301,9 @@ __device__ void diag_block_inverses(const T *Pn, T *Linv, int nq, int wave,
 /* ========================================================================
  * Fast path: no row interchanges, verified afterwards.
  * ====================================================================== */
-template <typename T, int NB, int MODE, int RPT>
-__global__ __launch_bounds__(512) void k_lu_fast(LuArgs a) {
+template <typename T, int NB, int MODE, int RPT, int NT>
+__global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
   typedef typename Vec4<T>::type V4;
-  constexpr int NT = 512;
   constexpr int NW = NT / 64;
   constexpr int PS = panel_stride(NB, MODE);
   constexpr int NQ = NB / 16;
@@ -928,14 +927,26 @@ int launch_kernel(Kern kern, size_t lds, int grid, int threads, const LuArgs &a,
 
 template <typename T, int NB, int MODE>
 int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
+  if (plan.threads == 256) {
+    /* two workgroups per CU; rows per thread up to 2 (M <= 512) */
+    if constexpr (NB <= 32) {
+      if (a.M <= 256)
+        return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256>, plan.lds, plan.fast_grid, 256, a, st);
+      if constexpr (!(sizeof(T) == 8 && NB == 16)) {
+        if (a.M <= 512)
+          return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256>, plan.lds, plan.fast_grid, 256, a, st);
+      }
+    }
+    return SGDLM_ERR_ARG;
+  }
   if (a.M <= 512)
-    return launch_kernel(k_lu_fast<T, NB, MODE, 1>, plan.lds, plan.grid, 512, a, st);
+    return launch_kernel(k_lu_fast<T, NB, MODE, 1, 512>, plan.lds, plan.fast_grid, 512, a, st);
   /* FP64 with 16-wide panels and two rows per thread is never planned (see
    * lu_plan); not instantiating it also avoids a register-coalescer crash of
    * the ROCm 7.2 compiler on that instantiation. */
   if constexpr (!(sizeof(T) == 8 && NB == 16)) {
     if (a.M <= 1024)
-      return launch_kernel(k_lu_fast<T, NB, MODE, 2>, plan.lds, plan.grid, 512, a, st);
+      return launch_kernel(k_lu_fast<T, NB, MODE, 2, 512>, plan.lds, plan.fast_grid, 512, a, st);
   }
   return SGDLM_ERR_ARG;
 }
@@ -1016,7 +1027,7 @@ LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
 } // namespace
 
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
-            LuPlan *plan) {
+            int want_threads, int want_nb, LuPlan *plan) {
   const int M = S.M;
   const int M2 = S.e_M2;
   const size_t ts = (prec == SGDLM_PREC_FP64) ? 8 : 4;
@@ -1025,32 +1036,55 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   plan->nb = 0;
   plan->pivot_nb = 0;
   plan->fast_M = M2;
+  plan->grid = n_cu > 0 ? n_cu : 256;
   const size_t slot_bytes = ((size_t)S.e_ns * ts + 15) & ~(size_t)15;
   const size_t pre_bytes = slot_bytes + (size_t)S.e_tab_words * 4;
-  for (int i = 0; i < 3 && !plan->nb && !solve; ++i) {
-    const int nb = cands[i];
-    if (prec == SGDLM_PREC_FP64 && nb > 32)
+  /* Fast-kernel configurations in order of preference: 256 threads with two
+   * workgroups per CU (LDS within half a CU, M <= 512), then 512 threads
+   * with one workgroup per CU. */
+  for (int pass = 0; pass < 2 && !plan->nb && !solve; ++pass) {
+    const int threads = pass == 0 ? 256 : 512;
+    const int per_cu = pass == 0 ? 2 : 1;
+    if (want_threads && want_threads != threads)
       continue;
-    if (prec == SGDLM_PREC_BF16X3 && nb < 32)
+    if (threads == 256 && M2 > 512)
       continue;
-    const size_t dense =
-        fast_lds_bytes(M2, nb, prec == SGDLM_PREC_BF16X3 ? MODE_BF16X3 : MODE_NATIVE, ts);
-    const bool fp64_narrow_wide = prec == SGDLM_PREC_FP64 && nb == 16 && M2 > 512;
-    if (M2 > 1024 || fp64_narrow_wide)
-      continue;
-    if (dense + pre_bytes <= lds_max) {
-      /* prefix region after the dense one: tables loaded once per workgroup */
+    const size_t cap = lds_max / per_cu;
+    for (int i = 0; i < 3 && !plan->nb; ++i) {
+      const int nb = cands[i];
+      if (want_nb && want_nb != nb)
+        continue;
+      if (prec == SGDLM_PREC_FP64 && nb > 32)
+        continue;
+      if (prec == SGDLM_PREC_BF16X3 && nb < 32)
+        continue;
+      if (threads == 256 && nb > 32)
+        continue;
+      const bool fp64_narrow_wide =
+          prec == SGDLM_PREC_FP64 && nb == 16 && M2 > (threads == 256 ? 256 : 512);
+      if (M2 > 1024 || fp64_narrow_wide)
+        continue;
+      const size_t dense =
+          fast_lds_bytes(M2, nb, prec == SGDLM_PREC_BF16X3 ? MODE_BF16X3 : MODE_NATIVE, ts);
+      if (dense + pre_bytes <= cap) {
+        /* prefix region after the dense one: tables loaded once per workgroup */
+        plan->lds = dense + pre_bytes;
+        plan->pre_base = dense;
+        plan->tab_persist = 1;
+      } else if (std::max(dense, pre_bytes) <= cap) {
+        plan->lds = std::max(dense, pre_bytes);
+        plan->pre_base = 0;
+        plan->tab_persist = 0;
+      } else {
+        continue;
+      }
       plan->nb = nb;
-      plan->lds = dense + pre_bytes;
-      plan->pre_base = dense;
-      plan->tab_persist = 1;
-    } else if (std::max(dense, pre_bytes) <= lds_max) {
-      plan->nb = nb;
-      plan->lds = std::max(dense, pre_bytes);
-      plan->pre_base = 0;
-      plan->tab_persist = 0;
+      plan->threads = threads;
+      plan->fast_grid = plan->grid * per_cu;
     }
   }
+  if (!solve && (want_threads || want_nb) && !plan->nb)
+    return SGDLM_ERR_ARG;
   for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
     const int nb = cands[i];
     if (prec == SGDLM_PREC_FP64 && nb > 32)
@@ -1063,8 +1097,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   }
   if (plan->pivot_nb == 0)
     return SGDLM_ERR_ARG;
-  plan->grid = n_cu > 0 ? n_cu : 256;
-  plan->work_bytes = (size_t)plan->grid * M * M * ts;
+  plan->work_bytes = (size_t)std::max(plan->grid, plan->fast_grid) * M * M * ts;
   return SGDLM_OK;
 }
 

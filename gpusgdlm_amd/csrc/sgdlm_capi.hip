@@ -63,6 +63,7 @@ struct sgdlm_ctx {
 
   int prec = SGDLM_PREC_FP64;
   int max_front = 64; /* sparse LU prefix: largest pivot degree (0 = off) */
+  int lu_threads = 0, lu_nb = 0; /* fast LU kernel tuning (0 = automatic) */
   uint64_t seed = 1234ull;
   uint32_t epoch = 0;
 
@@ -384,7 +385,8 @@ int build_structure(sgdlm_ctx *c) {
 }
 
 int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
-  int rc = lu_plan(c->S, prec, solve, c->n_cu, c->lds_max, plan);
+  int rc = lu_plan(c->S, prec, solve, c->n_cu, c->lds_max, solve ? 0 : c->lu_threads,
+                   solve ? 0 : c->lu_nb, plan);
   if (rc)
     return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
   return dalloc(c, c->sim, "lu_work", plan->work_bytes);
@@ -1075,22 +1077,35 @@ int sgdlm_get_lu_fallback_count(const sgdlm_ctx *cc, long *count) {
   return SGDLM_OK;
 }
 
-int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb) {
+int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, int *threads) {
   sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
   if (!c)
     return SGDLM_ERR_ARG;
   if (!c->parents_set)
     return SGDLM_ERR_STATE;
   LuPlan plan;
-  int rc = lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, &plan);
+  int rc = lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, c->lu_threads, c->lu_nb, &plan);
   if (rc)
-    return rc;
+    return fail(c, rc, "no LU configuration for threads=%d nb=%d", c->lu_threads, c->lu_nb);
   if (grid)
-    *grid = plan.grid;
+    *grid = plan.nb ? plan.fast_grid : plan.grid;
   if (lds)
     *lds = plan.lds;
   if (nb)
     *nb = plan.nb;
+  if (threads)
+    *threads = plan.nb ? plan.threads : 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_lu_tuning(sgdlm_ctx *c, int threads, int nb) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if ((threads != 0 && threads != 256 && threads != 512) ||
+      (nb != 0 && nb != 16 && nb != 32 && nb != 64))
+    return fail(c, SGDLM_ERR_ARG, "threads must be 0, 256 or 512 and nb 0, 16, 32 or 64");
+  c->lu_threads = threads;
+  c->lu_nb = nb;
   return SGDLM_OK;
 }
 

@@ -74,15 +74,19 @@ struct LuPlan {
   int fast_M = 0;             /* dense dimension of the fast kernel (the
                                  Schur complement after the sparse prefix)  */
   int pivot_nb = 0;           /* panel width, partial-pivoting kernel       */
-  int grid = 0;               /* persistent workgroups                      */
+  int grid = 0;               /* persistent workgroups (pivoting kernel)    */
+  int threads = 512;          /* workgroup size of the fast kernel          */
+  int fast_grid = 0;          /* persistent workgroups of the fast kernel   */
   size_t lds = 0;             /* dynamic LDS bytes, fast kernel             */
   size_t pre_base = 0;        /* LDS byte offset of the prefix slots/tables */
   int tab_persist = 0;        /* prefix tables stay in LDS across draws     */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
-  size_t work_bytes = 0;      /* global workspace bytes (grid * M * M * T)  */
+  size_t work_bytes = 0;      /* global workspace bytes                      */
 };
+/* want_threads / want_nb: 0 = automatic, else the fast kernel's workgroup
+ * size (256 or 512) and panel width; an infeasible request fails. */
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
-            LuPlan *plan);
+            int want_threads, int want_nb, LuPlan *plan);
 /* flags: N + 1 ints of scratch (count + list of draws handed to the
  * partial-pivoting kernel). */
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,

@@ -348,9 +348,14 @@ class SGDLM:
         return n.value
 
     def lu_config(self):
-        g, l, nb = C.c_int(), C.c_size_t(), C.c_int()
-        self._call("sgdlm_get_lu_config", C.byref(g), C.byref(l), C.byref(nb))
-        return {"grid": g.value, "lds": l.value, "nb": nb.value}
+        g, l, nb, th = C.c_int(), C.c_size_t(), C.c_int(), C.c_int()
+        self._call("sgdlm_get_lu_config", C.byref(g), C.byref(l), C.byref(nb), C.byref(th))
+        return {"grid": g.value, "lds": l.value, "nb": nb.value, "threads": th.value}
+
+    def set_lu_tuning(self, threads: int = 0, nb: int = 0):
+        """Pin the fast LU kernel's workgroup size (256 / 512) and panel width
+        (16 / 32 / 64); 0 = automatic."""
+        self._call("sgdlm_set_lu_tuning", C.c_int(int(threads)), C.c_int(int(nb)))
 
     def set_lu_max_front(self, max_front: int):
         """Largest pivot degree of the sparse LU prefix (0 = dense LU only)."""

@@ -155,9 +155,15 @@ int sgdlm_get_timing(const sgdlm_ctx *ctx, const char *stage, double *ms,
 /* Number of draws of the last VB call whose LU needed row interchanges
  * (factored by the partial-pivoting kernel instead of the fast path). */
 int sgdlm_get_lu_fallback_count(const sgdlm_ctx *ctx, long *count);
-/* Number of workgroups / LDS bytes / panel width chosen for the VB LU. */
+/* Workgroups / LDS bytes / panel width / workgroup size of the fast VB LU
+ * kernel for the current structure and precision (nb = 0: every draw goes
+ * through the partial-pivoting kernel). */
 int sgdlm_get_lu_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
-                        int *nb);
+                        int *nb, int *threads);
+/* Pins the fast VB LU kernel's workgroup size (256: two workgroups per CU,
+ * 512: one) and panel width (16, 32, 64); 0 = automatic.  A configuration
+ * that does not fit makes the next VB call fail with SGDLM_ERR_ARG. */
+int sgdlm_set_lu_tuning(sgdlm_ctx *ctx, int threads, int nb);
 /* Sparse prefix of the VB log-determinant LU: series whose degree in the
  * (symmetrised) parental graph stays <= max_front are eliminated by a
  * minimum-degree schedule before the dense LU of the remaining series

@@ -252,3 +252,33 @@ def test_logdet_sparse_prefix_m1000(hiplib, prec, tol):
     print(f"  {el}: max |err| {np.max(np.abs(lw - ld)):.3e}, fallback {fb}")
     assert np.max(np.abs(lw - ld)) <= tol
     assert fb <= N // 8
+
+
+@pytest.mark.parametrize("prec,cfgs,tol", [
+    ("fp32", [(256, 32), (256, 16), (512, 64), (512, 32)], 5e-3),
+    ("fp64", [(512, 32), (512, 16)], 1e-10),
+])
+def test_logdet_lu_tuning_configs(hiplib, prec, cfgs, tol):
+    """Every fast-kernel configuration (workgroup size x panel width) gives
+    the oracle's log-determinants on the same draws."""
+    mod = random_model(400, 20, T=45, seed=9)
+    N = 96
+    out = []
+    for th, nb in cfgs:
+        print(f"lu tuning {prec} threads={th} nb={nb}", flush=True)
+        g, o = make_pair(mod, prec)
+        g.set_lu_tuning(th, nb)
+        filter_steps(g, mod, 1, 40)
+        g.computePosterior(mod.y[40], mod.F(40))
+        g.computeVBPosterior(N, N)
+        cfg = g.lu_config()
+        assert (cfg["threads"], cfg["nb"]) == (th, nb)
+        out.append(g.get_vb_result()["log_weights"])
+        g.close()
+    filter_steps(o, mod, 1, 40)
+    o.computePosterior(mod.y[40], mod.F(40))
+    prm = o.getParameters()
+    lam, th_ = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th_)
+    for lw in out:
+        assert np.max(np.abs(lw - ld)) <= tol
