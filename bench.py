@@ -7,9 +7,10 @@ in HBM: Philox lambda/theta draws, the batched LU log-determinants of
 I - Gamma_i, log-sum-exp importance weights, weighted moments and the
 digamma/trigamma moment-match.  Results stay on the device (no PCIe copy in
 the timed region); the PCIe-inclusive rate through the full API is reported
-separately as `api_steps_per_s`.
+separately as `api_steps_per_s`.  Also reported: 1- and 10-step forecasts
+with N=1e5 draws (device-resident), and the FP64 oracle on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -20,23 +21,26 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "VB steps/s + IS draws/s at m=400,N=5e4: FP32-MFMA mixed vs plain FP64"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lu_fast_vb_m400_n50k_traffic.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def lu_flops(m: int) -> float:
-    """Algorithmic flops of one dense LU of I - Gamma (m x m)."""
-    return 2.0 / 3.0 * m ** 3
+def dense_lu_flops(n: int) -> float:
+    """Flops of one dense LU of an n x n matrix."""
+    return 2.0 / 3.0 * n ** 3
 
 
-def run_gpu(mod, prec, N, steps, warmup, T_filter):
+def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps):
     from gpusgdlm_amd import SGDLM
     from gpusgdlm_amd.workloads import configure, filter_steps
     g = SGDLM(strict=True)
@@ -48,29 +52,48 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter):
     for _ in range(warmup):
         g.vb_device(N, N)
     g.set_timing(True)
-    t0 = time.perf_counter()
+    times = []
     for _ in range(steps):
-        g.vb_device(N, N)
-    dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g.vb_device(N, N)          # synchronous: returns when the step is done
+        times.append(time.perf_counter() - t0)
     stage = {s: g.get_timing(s) for s in ("draw", "lu", "weights", "moments")}
     g.set_timing(False)
     res = g.get_vb_result()
-    fallback = g.lu_fallback_count()
+    out = {"ms_per_step": 1e3 * float(np.mean(times)), "median_ms": 1e3 * float(np.median(times)),
+           "p99_ms": 1e3 * float(np.percentile(times, 99)),
+           "stage_ms": {k: v[0] / max(v[1], 1) for k, v in stage.items()},
+           "ess": res["ess"], "lu_fallback_draws": g.lu_fallback_count(),
+           "lu_config": g.lu_config(), "elimination": g.lu_elimination()}
     # PCIe-inclusive figure through the full API (outputs copied to host)
     t1 = time.perf_counter()
-    out = g.computeVBPosterior(N, N)
-    api = time.perf_counter() - t1
-    cfg = g.lu_config()
+    g.computeVBPosterior(N, N)
+    out["api_ms"] = 1e3 * (time.perf_counter() - t1)
+    if fc_N:
+        g.computePrior()
+        Ff = mod.F_forecast(T_filter + 1)
+        g.forecast_device(fc_N, Ff, 1)          # warm-up (allocation, plan)
+        g.set_timing(True)
+        t0 = time.perf_counter()
+        g.forecast_device(fc_N, Ff, 1)
+        f1 = time.perf_counter() - t0
+        fl = g.get_timing("forecast_lu")
+        t0 = time.perf_counter()
+        g.forecast_device(fc_N, Ff, fc_steps)
+        fk = time.perf_counter() - t0
+        g.set_timing(False)
+        out["forecast"] = {"nsim": fc_N, "1step_ms": 1e3 * f1, f"{fc_steps}step_ms": 1e3 * fk,
+                           "lu_solve_ms_per_step": fl[0] / max(fl[1], 1)}
     g.close()
-    return {"ms_per_step": 1e3 * dt / steps, "stage_ms": {k: v[0] / max(v[1], 1) for k, v in stage.items()},
-            "lu_launches": stage["lu"][1], "ess": res["ess"], "lu_fallback_draws": fallback, "api_ms": 1e3 * api, "lu_config": cfg,
-            "vb": out}
+    return out
 
 
-def cpu_baseline(mod, T_filter, budget_s=12.0):
+def cpu_baseline(mod, T_filter, N_cap, budget_s=12.0):
     """FP64 oracle (OpenMP over draws) on a bounded sample of the same VB step:
     the LU log-determinants dominate (>99% of flops), so the sample is the
-    draws + LU of n_cpu importance draws, scaled to one VB step of N draws."""
+    draws + LU of n_cpu importance draws, scaled to one VB step of N draws.
+    The sample grows until it takes a quarter of the budget or covers the
+    whole step.  Also a 1-thread figure on 256 draws."""
     from oracle import oracle as orc
     from gpusgdlm_amd.workloads import configure, filter_steps
     o = orc.OracleSGDLM(seed=1234)
@@ -79,28 +102,52 @@ def cpu_baseline(mod, T_filter, budget_s=12.0):
     o.computePosterior(mod.y[T_filter], mod.F(T_filter))
     prm = o.getParameters()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    n = max(threads, 8)
-    while True:
+
+    def timed(n, nthreads):
         t0 = time.perf_counter()
         lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], n, 1234, 1)
-        orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th, nthreads=threads)
-        dt = time.perf_counter() - t0
-        if dt > budget_s / 4 or n >= 4096:
+        orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th, nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    n = max(threads, 8)
+    while True:
+        dt = timed(n, threads)
+        if dt > budget_s / 4 or n >= N_cap:
             break
-        n = int(n * min(8.0, max(2.0, budget_s / 4 / max(dt, 1e-3))))
-    return n, dt, threads
+        n = min(N_cap, int(n * min(8.0, max(2.0, budget_s / 4 / max(dt, 1e-3)))))
+    n1 = 256
+    dt1 = timed(n1, 1)
+    return {"n": n, "dt": dt, "threads": threads, "n1": n1, "dt1": dt1}
+
+
+def lu_traffic(prec, m, N, cfg):
+    """HBM bytes per k_lu_fast launch from the committed counter profile of
+    the same workload and kernel configuration (None if not measured)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("precisions", {}).get(prec)
+    if not e or t.get("m") != m or t.get("nsim") != N:
+        return None
+    if e.get("threads") != cfg["threads"] or e.get("nb") != cfg["nb"]:
+        return None
+    return e["fetch_bytes"] + e["write_bytes"]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nsim", type=int, default=50000)
     ap.add_argument("--m", type=int, default=400)
     ap.add_argument("--kmax", type=int, default=20)
     ap.add_argument("--filter-steps", type=int, default=250)
     ap.add_argument("--precisions", default="fp32,fp64,bf16x3")
+    ap.add_argument("--forecast-nsim", type=int, default=100000)
+    ap.add_argument("--forecast-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (counter calibration)")
@@ -127,15 +174,23 @@ def main():
     results = {}
     for prec in a.precisions.split(","):
         log(f"[{prec}] {a.warmup} warm-up + {a.steps} timed VB steps, N={N}")
-        results[prec] = run_gpu(mod, prec, N, a.steps, a.warmup, a.filter_steps)
-        log(f"[{prec}] {results[prec]['ms_per_step']:.2f} ms/step stages={results[prec]['stage_ms']} "
-            f"fallback={results[prec]['lu_fallback_draws']}")
+        fcN = a.forecast_nsim if prec in ("fp32", "fp64") else 0
+        results[prec] = run_gpu(mod, prec, N, a.steps, a.warmup, a.filter_steps, fcN, a.forecast_steps)
+        r = results[prec]
+        log(f"[{prec}] {r['ms_per_step']:.2f} ms/step (median {r['median_ms']:.2f}, p99 {r['p99_ms']:.2f}) "
+            f"stages={ {k: round(v, 2) for k, v in r['stage_ms'].items()} } fallback={r['lu_fallback_draws']} "
+            f"forecast={r.get('forecast')}")
 
     head = "fp32" if "fp32" in results else next(iter(results))
     r = results[head]
     steps_s = 1e3 / r["ms_per_step"]
-    lu_ms = r["stage_ms"]["lu"]
-    achieved = lu_flops(a.m) * N / (lu_ms * 1e-3) / 1e12
+    lu_s = r["stage_ms"]["lu"] * 1e-3
+    el = r["elimination"]
+    dense_eq = dense_lu_flops(a.m) * N                      # per launch, as the reference computes it
+    M2 = (el["n_dense"] + 15) // 16 * 16
+    executed = (dense_lu_flops(M2) + 2.0 * el["prefix_fma"]) * N
+    achieved = dense_eq / lu_s / 1e12
+    traffic = lu_traffic(head, a.m, N, r["lu_config"])
     line = {
         "metric": METRIC,
         "value": steps_s,
@@ -144,33 +199,41 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": r["ms_per_step"],
+        "median_ms": r["median_ms"],
+        "p99_ms": r["p99_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (LU on MFMA) + f64 (draws, log-det sums, weights, moments)",
+        "dtype": "f32+f64",
         "data": "synthetic: random SGDLM (seed 20250101), 250 naive filter steps, then VB",
         "config": {"workload": f"VB step, m={a.m} series, k~U{{0..{a.kmax}}} simultaneous parents, "
-                               f"max_p={mod.max_p}, N={N} IS draws",
-                   "m": a.m, "max_p": mod.max_p, "nsim": N, "lu": r["lu_config"]},
+                               f"max_p={mod.max_p}, N={N} IS draws, LU in FP32 on MFMA",
+                   "m": a.m, "max_p": mod.max_p, "nsim": N, "lu": r["lu_config"], "elimination": el},
         "is_draws_per_s": steps_s * N,
         "variants": {p: {"vb_steps_per_s": 1e3 / v["ms_per_step"], "is_draws_per_s": 1e3 / v["ms_per_step"] * N,
-                         "ms_per_step": v["ms_per_step"], "stage_ms": v["stage_ms"], "ess": v["ess"],
-                         "lu_fallback_draws": v["lu_fallback_draws"],
-                         "api_steps_per_s": 1e3 / v["api_ms"]}
+                         "ms_per_step": v["ms_per_step"], "median_ms": v["median_ms"], "p99_ms": v["p99_ms"],
+                         "stage_ms": v["stage_ms"], "ess": v["ess"],
+                         "lu_fallback_draws": v["lu_fallback_draws"], "lu": v["lu_config"],
+                         "api_steps_per_s": 1e3 / v["api_ms"], "forecast": v.get("forecast")}
                      for p, v in results.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[head], "traffic": None,
-                     "kernel": f"k_lu ({head}), {lu_flops(a.m) * N / 1e12:.3f} TFLOP algorithmic per launch"},
+                     "frac": achieved / PEAK[head], "traffic": traffic,
+                     "kernel": f"k_lu_fast ({head}); achieved = dense-equivalent (2/3) m^3 x N = "
+                               f"{dense_eq / 1e12:.3f} TFLOP per launch / LU time",
+                     "executed_tflops": executed / lu_s / 1e12,
+                     "executed_tflop_per_launch": executed / 1e12},
     }
     if "fp64" in results and head != "fp64":
         line["speedup_vs_fp64"] = results["fp64"]["ms_per_step"] / r["ms_per_step"]
     if not a.no_cpu:
-        n, dt, thr = cpu_baseline(mod, a.filter_steps)
-        cpu_steps = 1.0 / (dt * N / n)
-        line["cpu_baseline"] = {"value": cpu_steps, "unit": "VB steps/s", "cores": thr, "kind": "port",
-                                "sample": f"FP64 oracle draws + LU log-dets of {n} of the {N} draws "
-                                          f"({dt:.2f} s), scaled to one VB step",
-                                "is_draws_per_s": n / dt}
+        cb = cpu_baseline(mod, a.filter_steps, N)
+        cpu_steps = 1.0 / (cb["dt"] * N / cb["n"])
+        line["cpu_baseline"] = {"value": cpu_steps, "unit": "VB steps/s", "cores": cb["threads"], "kind": "port",
+                                "sample": f"FP64 oracle (OpenMP) draws + LU log-dets of {cb['n']} of the {N} draws "
+                                          f"({cb['dt']:.2f} s), scaled to one VB step",
+                                "is_draws_per_s": cb["n"] / cb["dt"],
+                                "one_core_vb_steps_per_s": 1.0 / (cb["dt1"] * N / cb["n1"]),
+                                "one_core_sample": f"{cb['n1']} draws in {cb['dt1']:.2f} s on 1 thread"}
     print(json.dumps(line), flush=True)
 
 

@@ -320,6 +320,23 @@ class SGDLM:
             return False
         return self._call("sgdlm_compute_vb_posterior", None, None, None, None, None, None)
 
+    def forecast_device(self, nsim, F_tp1, nsteps=1):
+        """computeForecast's device work (draws, evolution for k > 0, LU
+        solves) for nsteps steps ahead with the same F_tp1 at every step;
+        the forecasts stay on the device (no PCIe copy)."""
+        F = _f(F_tp1)
+        if not self._ensure_sim(int(nsim), int(nsim)):
+            return False
+        if not self._call("sgdlm_init_forecast_priors"):
+            return False
+        for k in range(int(nsteps)):
+            if k > 0 and not self._call("sgdlm_evolve_forecast_samples", None):
+                return False
+            if not self._call("sgdlm_compute_forecast", None, nat.dptr(F), None, None,
+                              C.c_int(1 if k > 0 else 0)):
+                return False
+        return True
+
     def get_vb_result(self):
         m, mp = self._dims()
         n, _ = self._nsim()
