@@ -244,3 +244,46 @@ def test_multistep_evolution_identities(oracle_lib):
     r = o.computeForecastDebug(300, 300, Fk)
     assert np.array_equal(r["lambdas"][:, :, 0], r["lambdas"][:, :, 1])
     assert np.allclose(r["thetas"][:, :, :, 0], r["thetas"][:, :, :, 1], rtol=0, atol=0)
+
+
+def test_theta_transform_moments(oracle_lib):
+    """sgo_sample: lambda ~ Gamma(n/2, rate n s/2) gives E[lambda] = 1/s,
+    and sqrt(s lambda) (theta - m) ~ N(0, C) -- a transposed Cholesky factor
+    (U U' instead of U'U), a missing sqrt or a dropped s fails this."""
+    rng = np.random.default_rng(4)
+    P, mp = 3, 4
+    X = rng.normal(size=(P, P + 2))
+    Cb = X @ X.T / 4 + 0.2 * np.eye(P)
+    Cb[0, 2] = Cb[2, 0] = 0.6 * math.sqrt(Cb[0, 0] * Cb[2, 2])
+    mt, Ct, nt, st, p = one_series([0.3, -1.2, 2.0], Cb, 12.0, 0.7, p=P, max_p=mp)
+    N = 60000
+    lam, th = orc.sample(1, mp, p, mt, Ct, nt, st, N, 77, 5)
+    assert np.mean(lam) == pytest.approx(1 / 0.7, rel=0.01)
+    z = (th[:P, 0, :] - mt[:P, :1]) * np.sqrt(0.7 * lam[0])
+    assert np.allclose(np.cov(z), Cb, rtol=0.03, atol=0.03 * np.max(np.abs(Cb)))
+    assert np.all(th[P:, 0, :] == mt[P:, :1])
+
+
+def test_evolution_moments(oracle_lib):
+    """sgo_evolve: lambda' = lambda * eta with E[eta] = 1 (Beta(beta r/2,
+    (1-beta) r/2) / beta), and theta' - theta = chol(W)' omega / sqrt(c
+    lambda') ~ N(0, W / (c lambda')) on the p_j block, zero beyond it."""
+    rng = np.random.default_rng(6)
+    P, mp, N = 3, 4, 60000
+    X = rng.normal(size=(P, P + 2))
+    Wb = X @ X.T / 5 + 0.1 * np.eye(P)
+    W = np.zeros((mp, mp, 1), order="F")
+    W[:P, :P, 0] = Wb
+    lam = np.full((1, N), 2.0, order="F")
+    th = np.zeros((mp, 1, N), order="F")
+    r_t, c_t, beta = np.array([20.0]), np.array([0.5]), np.array([0.9])
+    orc.lib().sgo_evolve(1, mp, orc.u(np.array([P], np.uint32)), orc.d(W), orc.d(r_t), orc.d(c_t),
+                         orc.d(beta), None, N, 11, 3, orc.d(lam), orc.d(th))
+    eta = lam[0] / 2.0
+    assert np.mean(eta) == pytest.approx(1.0, rel=0.005)
+    a, b = 0.9 * 20 / 2, 0.1 * 20 / 2            # Beta(a, b) / beta
+    var_eta = a * b / ((a + b) ** 2 * (a + b + 1)) / 0.81
+    assert np.var(eta) == pytest.approx(var_eta, rel=0.05)
+    z = th[:P, 0, :] * np.sqrt(0.5 * lam[0])
+    assert np.allclose(np.cov(z), Wb, rtol=0.03, atol=0.03 * np.max(np.abs(Wb)))
+    assert np.all(th[P:, 0, :] == 0)
