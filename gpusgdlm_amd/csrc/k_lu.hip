@@ -301,8 +301,9 @@ __device__ void diag_block_inverses(const T *Pn, T *Linv, int nq, int wave,
 /* ========================================================================
  * Fast path: no row interchanges, verified afterwards.
  * ====================================================================== */
-template <typename T, int NB, int MODE, int RPT, int NT>
+template <typename T, int NB, int MODE, int RPT, int NT, int NBI>
 __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
+  static_assert(NB % NBI == 0 && (MODE == MODE_NATIVE || NBI == NB), "sub-panel width");
   typedef typename Vec4<T>::type V4;
   constexpr int NW = NT / 64;
   constexpr int PS = panel_stride(NB, MODE);
@@ -337,196 +338,106 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     prefix_assemble<T>(A, M, smem, a.theta + (size_t)d * a.P, a, tid, NT,
                        !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls);
 
-    for (int K = 0; K < M; K += NB) {
-      const int jb = min(NB, M - K);
-      const int nr = M - K;
-      const int nq = jb >> 4;
-
-      /* ---- panel rows into registers (16-byte loads) ---- */
-      T v[RPT][NB];
+    /* ---- phase A: U12 = L11^{-1} A12 for the column tiles
+     * [c_first, c_first + 16 ntc), L11 = the nqp 16-row blocks of the panel
+     * (Pn rows 0..16 nqp, Linv), written in place of A12 ---- */
+    auto phase_a = [&](int Kp, int nqp, int c_first, int ntc) {
+      for (int ct = wave; ct < ntc; ct += NW) {
+        const int c0 = c_first + 16 * ct;
+        V4 u[NQ];
 #pragma unroll
-      for (int rr = 0; rr < RPT; ++rr) {
-        const int r = tid + NT * rr;
-        const VV *src = reinterpret_cast<const VV *>(A + (size_t)row_of(K + r < M ? K + r : 0) * M + K);
+        for (int q = 0; q < NQ; ++q) {
+          V4 acc = {0, 0, 0, 0};
+          if (q < nqp) {
 #pragma unroll
-        for (int c4 = 0; c4 < NB / VW; ++c4) {
-          VV x = {};
-          if (r < nr && c4 * VW < jb)
-            x = src[c4];
-#pragma unroll
-          for (int e = 0; e < VW; ++e)
-            v[rr][c4 * VW + e] = x[e];
+            for (int r = 0; r < 4; ++r)
+              acc[r] = A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+          }
+          u[q] = acc;
         }
-      }
-
-      /* ---- factor without interchanges: one barrier per column ---- */
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        if (j < jb) {
-          T *pr = prow + (j & 1) * NB;
-          const int c4s = j / VW; /* first 16-byte chunk holding column j */
-          if (tid == j) {
+        for (int q = 0; q < NQ; ++q) {
+          if (q < nqp) {
+            V4 acc = u[q];
 #pragma unroll
-            for (int c4 = c4s; c4 < NB / VW; ++c4) {
-              VV x;
+            for (int aa = 0; aa < NQ; ++aa)
+              if (aa < q)
 #pragma unroll
-              for (int e = 0; e < VW; ++e)
-                x[e] = v[0][c4 * VW + e];
-              reinterpret_cast<VV *>(pr)[c4] = x;
-            }
-            dg[K + j] = v[0][j];
-          }
-          __syncthreads();
-          T dj = pr[j];
-          if (!(fabs(dj) > (T)0)) {
-            bad = 1;
-            dj = (T)1;
-          }
+                for (int r = 0; r < 4; ++r)
+                  acc = mfma16(-Pn[(16 * q + l16) * PS + 16 * aa + rho<T>(g, r)], u[aa][r], acc);
+            V4 uq = {0, 0, 0, 0};
 #pragma unroll
-          for (int rr = 0; rr < RPT; ++rr) {
-            const int r = tid + NT * rr;
-            if (r > j && r < nr) {
-              const T l = v[rr][j] / dj;
-              v[rr][j] = l;
-              amax = fmax(amax, fabs(l));
+            for (int r = 0; r < 4; ++r)
+              uq = mfma16(Linv[q * 256 + l16 * 16 + rho<T>(g, r)], acc[r], uq);
+            u[q] = uq;
 #pragma unroll
-              for (int c4 = c4s; c4 < NB / VW; ++c4) {
-                const VV x = reinterpret_cast<const VV *>(pr)[c4];
-#pragma unroll
-                for (int e = 0; e < VW; ++e)
-                  if (c4 * VW + e > j)
-                    v[rr][c4 * VW + e] = fma(-l, x[e], v[rr][c4 * VW + e]);
-              }
-            }
+            for (int r = 0; r < 4; ++r)
+              A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[q][r];
           }
         }
       }
+    };
 
-      /* ---- panel to LDS (bf16 mode: rows >= jb as -L21 hi/lo) ---- */
+    /* ---- phase B: A22 -= L21 U12 over the row tiles starting at
+     * Kp + jbp (ntr of them) and the column tiles [c_first, + 16 ntc);
+     * L21 = Pn rows jbp.., K-width 16 nqp.  Items are (pair of column tiles
+     * = 32 columns, row tile): full 128-byte rows for FP32, shared A
+     * operands; 8 contiguous item ranges; loads one batch ahead. ---- */
+    auto phase_b = [&](int Kp, int jbp, int nqp, int c_first, int ntc, int ntr, bool bf16) {
+      const int ntp = (ntc + 1) >> 1;
+      const long tot = (long)ntp * ntr;
+      const long lo_it = tot * wave / NW, hi_it = tot * (wave + 1) / NW;
+      int cur_cp = -1;
+      V4 ub[2][NQ];
+      u16x8 bhi[2][NKB], blo[2][NKB];
+      V4 cc[BATCH][2], cn[BATCH][2];
+      auto load_batch = [&](long it0, V4 (*dst)[2]) {
 #pragma unroll
-      for (int rr = 0; rr < RPT; ++rr) {
-        const int r = tid + NT * rr;
-        if (r < nr) {
-          if (MODE == MODE_BF16X3 && r >= jb) {
-            unsigned short *row = reinterpret_cast<unsigned short *>(Pn + r * PS);
+        for (int b = 0; b < BATCH; ++b) {
+          const long it = it0 + b;
+          if (it < hi_it) {
+            const int cp = (int)(it / ntr);
+            const int rt = (int)(it - (long)cp * ntr);
+            const int col = c_first + 32 * cp + l16;
 #pragma unroll
-            for (int kb = 0; kb < NKB; ++kb)
+            for (int r = 0; r < 4; ++r) {
+              const T *src = A + (size_t)row_of(Kp + jbp + 16 * rt + rho<T>(g, r)) * M + col;
 #pragma unroll
-              for (int gg = 0; gg < 4; ++gg)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                  const int col = 32 * kb + 16 * (e >> 2) + 4 * gg + (e & 3);
-                  const float x = (col < NB) ? -(float)v[rr][col < NB ? col : 0] : 0.f;
-                  const unsigned short hi = f2bf(x);
-                  row[(kb * 4 + gg) * 16 + e] = hi;
-                  row[(kb * 4 + gg) * 16 + 8 + e] = f2bf(x - bf2f(hi));
-                }
-          } else {
-#pragma unroll
-            for (int c = 0; c < NB; ++c)
-              Pn[r * PS + c] = v[rr][c];
-          }
-        }
-      }
-      __syncthreads();
-      diag_block_inverses<T, PS>(Pn, Linv, nq, wave, NW, lane);
-      __syncthreads();
-
-      const int ntc = (M - K - jb) >> 4;
-      if (ntc > 0) {
-        /* ---- phase A: U12 = L11^{-1} A12, written in place of A12 ---- */
-        for (int ct = wave; ct < ntc; ct += NW) {
-          const int c0 = K + jb + 16 * ct;
-          V4 u[NQ];
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            V4 acc = {0, 0, 0, 0};
-            if (q < nq) {
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-                acc[r] = A[(size_t)row_of(K + 16 * q + rho<T>(g, r)) * M + c0 + l16];
-            }
-            u[q] = acc;
-          }
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            if (q < nq) {
-              V4 acc = u[q];
-#pragma unroll
-              for (int aa = 0; aa < NQ; ++aa)
-                if (aa < q)
-#pragma unroll
-                  for (int r = 0; r < 4; ++r)
-                    acc = mfma16(-Pn[(16 * q + l16) * PS + 16 * aa + rho<T>(g, r)], u[aa][r], acc);
-              V4 uq = {0, 0, 0, 0};
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-                uq = mfma16(Linv[q * 256 + l16 * 16 + rho<T>(g, r)], acc[r], uq);
-              u[q] = uq;
-#pragma unroll
-              for (int r = 0; r < 4; ++r)
-                A[(size_t)row_of(K + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[q][r];
+              for (int h = 0; h < 2; ++h)
+                if (2 * cp + h < ntc)
+                  dst[b][h][r] = src[16 * h];
             }
           }
         }
-        __syncthreads();
-
-        /* ---- phase B: A22 -= L21 U12.  Items are (pair of column tiles =
-         * 32 columns, row tile): full 128-byte rows for FP32, shared A
-         * operands; 8 contiguous item ranges; loads one batch ahead. ---- */
-        const int ntp = (ntc + 1) >> 1;
-        const long tot = (long)ntp * ntc;
-        const long lo_it = tot * wave / NW, hi_it = tot * (wave + 1) / NW;
-        int cur_cp = -1;
-        V4 ub[2][NQ];
-        u16x8 bhi[2][NKB], blo[2][NKB];
-        V4 cc[BATCH][2], cn[BATCH][2];
-        auto load_batch = [&](long it0, V4 (*dst)[2]) {
+      };
+      if (lo_it < hi_it)
+        load_batch(lo_it, cc);
+      for (long it0 = lo_it; it0 < hi_it; it0 += BATCH) {
+        if (it0 + BATCH < hi_it)
+          load_batch(it0 + BATCH, cn);
 #pragma unroll
-          for (int b = 0; b < BATCH; ++b) {
-            const long it = it0 + b;
-            if (it < hi_it) {
-              const int cp = (int)(it / ntc);
-              const int rt = (int)(it - (long)cp * ntc);
-              const int col = K + jb + 32 * cp + l16;
+        for (int b = 0; b < BATCH; ++b) {
+          const long it = it0 + b;
+          if (it < hi_it) {
+            const int cp = (int)(it / ntr);
+            const int rt = (int)(it - (long)cp * ntr);
+            const bool two = 2 * cp + 1 < ntc;
+            if (cp != cur_cp) {
+              cur_cp = cp;
 #pragma unroll
-              for (int r = 0; r < 4; ++r) {
-                const T *src = A + (size_t)row_of(K + jb + 16 * rt + rho<T>(g, r)) * M + col;
+              for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-                  if (2 * cp + h < ntc)
-                    dst[b][h][r] = src[16 * h];
-              }
-            }
-          }
-        };
-        if (lo_it < hi_it)
-          load_batch(lo_it, cc);
-        for (long it0 = lo_it; it0 < hi_it; it0 += BATCH) {
-          if (it0 + BATCH < hi_it)
-            load_batch(it0 + BATCH, cn);
+                for (int q = 0; q < NQ; ++q) {
+                  V4 t = {0, 0, 0, 0};
+                  if (q < nqp && (h == 0 || two)) {
 #pragma unroll
-          for (int b = 0; b < BATCH; ++b) {
-            const long it = it0 + b;
-            if (it < hi_it) {
-              const int cp = (int)(it / ntc);
-              const int rt = (int)(it - (long)cp * ntc);
-              const bool two = 2 * cp + 1 < ntc;
-              if (cp != cur_cp) {
-                cur_cp = cp;
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                  for (int q = 0; q < NQ; ++q) {
-                    V4 t = {0, 0, 0, 0};
-                    if (q < nq && (h == 0 || two)) {
-#pragma unroll
-                      for (int r = 0; r < 4; ++r)
-                        t[r] = A[(size_t)row_of(K + 16 * q + rho<T>(g, r)) * M + K + jb + 32 * cp + 16 * h + l16];
-                    }
-                    ub[h][q] = t;
+                    for (int r = 0; r < 4; ++r)
+                      t[r] = A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c_first + 32 * cp + 16 * h + l16];
                   }
-                if constexpr (MODE == MODE_BF16X3) {
+                  ub[h][q] = t;
+                }
+              if constexpr (MODE == MODE_BF16X3) {
+                if (bf16) {
 #pragma unroll
                   for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -541,13 +452,16 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
                       }
                 }
               }
-              V4 c0v = cc[b][0], c1v = cc[b][1];
-              const int prw = jb + 16 * rt + l16;
-              if constexpr (MODE == MODE_BF16X3) {
+            }
+            V4 c0v = cc[b][0], c1v = cc[b][1];
+            const int prw = jbp + 16 * rt + l16;
+            bool done = false;
+            if constexpr (MODE == MODE_BF16X3) {
+              if (bf16) {
                 const char *rowp = reinterpret_cast<const char *>(Pn + prw * PS);
 #pragma unroll
                 for (int kb = 0; kb < NKB; ++kb) {
-                  if (2 * kb < nq) {
+                  if (2 * kb < nqp) {
                     const u16x8 ahi = *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32);
                     const u16x8 alo = *reinterpret_cast<const u16x8 *>(rowp + (kb * 4 + g) * 32 + 16);
                     c0v = mfma_bf16(alo, bhi[0][kb], c0v);
@@ -560,33 +474,157 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
                     }
                   }
                 }
-              } else {
-#pragma unroll
-                for (int aa = 0; aa < NQ; ++aa)
-                  if (aa < nq)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                      const T av = -Pn[prw * PS + 16 * aa + rho<T>(g, r)];
-                      c0v = mfma16(av, ub[0][aa][r], c0v);
-                      if (two)
-                        c1v = mfma16(av, ub[1][aa][r], c1v);
-                    }
+                done = true;
               }
+            }
+            if (!done) {
 #pragma unroll
-              for (int r = 0; r < 4; ++r) {
-                T *dst = A + (size_t)row_of(K + jb + 16 * rt + rho<T>(g, r)) * M + K + jb + 32 * cp + l16;
-                dst[0] = (T)c0v[r];
-                if (two)
-                  dst[16] = (T)c1v[r];
+              for (int aa = 0; aa < NQ; ++aa)
+                if (aa < nqp)
+#pragma unroll
+                  for (int r = 0; r < 4; ++r) {
+                    const T av = -Pn[prw * PS + 16 * aa + rho<T>(g, r)];
+                    c0v = mfma16(av, ub[0][aa][r], c0v);
+                    if (two)
+                      c1v = mfma16(av, ub[1][aa][r], c1v);
+                  }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              T *dst = A + (size_t)row_of(Kp + jbp + 16 * rt + rho<T>(g, r)) * M + c_first + 32 * cp + l16;
+              dst[0] = (T)c0v[r];
+              if (two)
+                dst[16] = (T)c1v[r];
+            }
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+          cc[b][0] = cn[b][0];
+          cc[b][1] = cn[b][1];
+        }
+      }
+    };
+
+    for (int K = 0; K < M; K += NB) {
+      const int jb = min(NB, M - K);
+      const int nq = jb >> 4;
+
+      /* ---- the panel in sub-panels of NBI columns: rows in registers,
+       * factored without interchanges, one barrier per column; between
+       * sub-panels the next sub-panel's columns get the previous one's
+       * update (phases A and B restricted to those columns, native MFMA) */
+      for (int Ks = K; Ks < K + jb; Ks += NBI) {
+        const int so = Ks - K;              /* sub-panel offset in the panel */
+        const int jbs = min(NBI, K + jb - Ks);
+        const int nr = M - Ks;
+        if (so > 0) {
+          diag_block_inverses<T, PS>(Pn, Linv, so >> 4, wave, NW, lane);
+          __syncthreads();
+          phase_a(K, so >> 4, Ks, jbs >> 4);
+          __syncthreads();
+          phase_b(K, so, so >> 4, Ks, jbs >> 4, (M - Ks) >> 4, false);
+          __syncthreads();
+        }
+
+        /* ---- sub-panel rows into registers (16-byte loads) ---- */
+        T v[RPT][NBI];
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+          const int r = tid + NT * rr;
+          const VV *src = reinterpret_cast<const VV *>(A + (size_t)row_of(Ks + r < M ? Ks + r : 0) * M + Ks);
+#pragma unroll
+          for (int c4 = 0; c4 < NBI / VW; ++c4) {
+            VV x = {};
+            if (r < nr && c4 * VW < jbs)
+              x = src[c4];
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+              v[rr][c4 * VW + e] = x[e];
+          }
+        }
+
+        /* ---- factor without interchanges: one barrier per column ---- */
+#pragma unroll
+        for (int j = 0; j < NBI; ++j) {
+          if (j < jbs) {
+            T *pr = prow + (j & 1) * NB;
+            const int c4s = j / VW; /* first 16-byte chunk holding column j */
+            if (tid == j) {
+#pragma unroll
+              for (int c4 = c4s; c4 < NBI / VW; ++c4) {
+                VV x;
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                  x[e] = v[0][c4 * VW + e];
+                reinterpret_cast<VV *>(pr)[c4] = x;
+              }
+              dg[Ks + j] = v[0][j];
+            }
+            __syncthreads();
+            T dj = pr[j];
+            if (!(fabs(dj) > (T)0)) {
+              bad = 1;
+              dj = (T)1;
+            }
+#pragma unroll
+            for (int rr = 0; rr < RPT; ++rr) {
+              const int r = tid + NT * rr;
+              if (r > j && r < nr) {
+                const T l = v[rr][j] / dj;
+                v[rr][j] = l;
+                amax = fmax(amax, fabs(l));
+#pragma unroll
+                for (int c4 = c4s; c4 < NBI / VW; ++c4) {
+                  const VV x = reinterpret_cast<const VV *>(pr)[c4];
+#pragma unroll
+                  for (int e = 0; e < VW; ++e)
+                    if (c4 * VW + e > j)
+                      v[rr][c4 * VW + e] = fma(-l, x[e], v[rr][c4 * VW + e]);
+                }
               }
             }
           }
+        }
+
+        /* ---- sub-panel to LDS (Pn rows relative to K; bf16 mode: rows
+         * >= jb of the whole panel as -L21 hi/lo, single sub-panel only) */
 #pragma unroll
-          for (int b = 0; b < BATCH; ++b) {
-            cc[b][0] = cn[b][0];
-            cc[b][1] = cn[b][1];
+        for (int rr = 0; rr < RPT; ++rr) {
+          const int r = tid + NT * rr;
+          if (r < nr) {
+            const int pr_ = so + r;
+            if (MODE == MODE_BF16X3 && pr_ >= jb) {
+              unsigned short *row = reinterpret_cast<unsigned short *>(Pn + pr_ * PS);
+#pragma unroll
+              for (int kb = 0; kb < NKB; ++kb)
+#pragma unroll
+                for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+                  for (int e = 0; e < 8; ++e) {
+                    const int col = 32 * kb + 16 * (e >> 2) + 4 * gg + (e & 3);
+                    const float x = (col < NBI) ? -(float)v[rr][col < NBI ? col : 0] : 0.f;
+                    const unsigned short hi = f2bf(x);
+                    row[(kb * 4 + gg) * 16 + e] = hi;
+                    row[(kb * 4 + gg) * 16 + 8 + e] = f2bf(x - bf2f(hi));
+                  }
+            } else {
+#pragma unroll
+              for (int c = 0; c < NBI; ++c)
+                Pn[pr_ * PS + so + c] = v[rr][c];
+            }
           }
         }
+        __syncthreads();
+      }
+      diag_block_inverses<T, PS>(Pn, Linv, nq, wave, NW, lane);
+      __syncthreads();
+
+      const int ntc = (M - K - jb) >> 4;
+      if (ntc > 0) {
+        phase_a(K, nq, K + jb, ntc);
+        __syncthreads();
+        phase_b(K, jb, nq, K + jb, ntc, ntc, MODE == MODE_BF16X3);
       }
       __syncthreads();
     }
@@ -927,26 +965,28 @@ int launch_kernel(Kern kern, size_t lds, int grid, int threads, const LuArgs &a,
 
 template <typename T, int NB, int MODE>
 int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
+  /* register sub-panels of 32 columns for 64-wide native panels */
+  constexpr int NBI = (MODE == MODE_NATIVE && NB > 32) ? 32 : NB;
   if (plan.threads == 256) {
     /* two workgroups per CU; rows per thread up to 2 (M <= 512) */
-    if constexpr (NB <= 32) {
+    if constexpr (NBI <= 32) {
       if (a.M <= 256)
-        return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256>, plan.lds, plan.fast_grid, 256, a, st);
+        return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI>, plan.lds, plan.fast_grid, 256, a, st);
       if constexpr (!(sizeof(T) == 8 && NB == 16)) {
         if (a.M <= 512)
-          return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256>, plan.lds, plan.fast_grid, 256, a, st);
+          return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256, NBI>, plan.lds, plan.fast_grid, 256, a, st);
       }
     }
     return SGDLM_ERR_ARG;
   }
   if (a.M <= 512)
-    return launch_kernel(k_lu_fast<T, NB, MODE, 1, 512>, plan.lds, plan.fast_grid, 512, a, st);
+    return launch_kernel(k_lu_fast<T, NB, MODE, 1, 512, NBI>, plan.lds, plan.fast_grid, 512, a, st);
   /* FP64 with 16-wide panels and two rows per thread is never planned (see
    * lu_plan); not instantiating it also avoids a register-coalescer crash of
    * the ROCm 7.2 compiler on that instantiation. */
   if constexpr (!(sizeof(T) == 8 && NB == 16)) {
     if (a.M <= 1024)
-      return launch_kernel(k_lu_fast<T, NB, MODE, 2, 512>, plan.lds, plan.fast_grid, 512, a, st);
+      return launch_kernel(k_lu_fast<T, NB, MODE, 2, 512, NBI>, plan.lds, plan.fast_grid, 512, a, st);
   }
   return SGDLM_ERR_ARG;
 }
@@ -954,6 +994,8 @@ int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
 int dispatch_fast(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   switch (plan.prec) {
   case SGDLM_PREC_FP64:
+    if (plan.nb == 64)
+      return fast_rpt<double, 64, MODE_NATIVE>(a, plan, st);
     if (plan.nb == 32)
       return fast_rpt<double, 32, MODE_NATIVE>(a, plan, st);
     return fast_rpt<double, 16, MODE_NATIVE>(a, plan, st);
@@ -1054,11 +1096,9 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
       const int nb = cands[i];
       if (want_nb && want_nb != nb)
         continue;
-      if (prec == SGDLM_PREC_FP64 && nb > 32)
-        continue;
       if (prec == SGDLM_PREC_BF16X3 && nb < 32)
         continue;
-      if (threads == 256 && nb > 32)
+      if (threads == 256 && nb > 32 && prec == SGDLM_PREC_BF16X3)
         continue;
       const bool fp64_narrow_wide =
           prec == SGDLM_PREC_FP64 && nb == 16 && M2 > (threads == 256 ? 256 : 512);

@@ -62,7 +62,7 @@ struct sgdlm_ctx {
   size_t nsim = 0, nsim_batch = 0;
 
   int prec = SGDLM_PREC_FP64;
-  int max_front = 64; /* sparse LU prefix: largest pivot degree (0 = off) */
+  int max_front = 32; /* sparse LU prefix: largest pivot degree (0 = off) */
   int lu_threads = 0, lu_nb = 0; /* fast LU kernel tuning (0 = automatic) */
   uint64_t seed = 1234ull;
   uint32_t epoch = 0;

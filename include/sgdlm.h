@@ -167,7 +167,7 @@ int sgdlm_set_lu_tuning(sgdlm_ctx *ctx, int threads, int nb);
 /* Sparse prefix of the VB log-determinant LU: series whose degree in the
  * (symmetrised) parental graph stays <= max_front are eliminated by a
  * minimum-degree schedule before the dense LU of the remaining series
- * (default 64; 0 = dense LU of the whole I - Gamma).  The result is the
+ * (default 32; 0 = dense LU of the whole I - Gamma).  The result is the
  * same determinant; only the rounding differs. */
 int sgdlm_set_lu_max_front(sgdlm_ctx *ctx, int max_front);
 /* The plan for the current parental sets: series eliminated in the prefix,
