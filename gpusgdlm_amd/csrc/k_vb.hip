@@ -134,46 +134,87 @@ __global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
 
 /* pass 1: per (chunk, series) sums of w*lambda*theta_k, w*lambda,
  * w*log(lambda).  threads = 4 draw groups x 64 coefficient lanes. */
-__global__ __launch_bounds__(256) void k_mom1(long N, int m, int max_p,
-                                              int chunks, const unsigned *p,
-                                              const int *off, long P,
-                                              const double *w,
-                                              const double *lam,
-                                              const double *theta,
-                                              double *part1) {
-  __shared__ double red[4][66];
-  const int j = blockIdx.y, c = blockIdx.x;
-  const long i0 = N * c / chunks, i1 = N * (c + 1) / chunks;
-  const int tid = threadIdx.x, kl = tid & 63, dg = tid >> 6;
-  const int pj = (int)p[j];
-  const double *th = theta + off[j];
-  double at = 0.0, al = 0.0, ll = 0.0;
-  for (long i = i0 + dg; i < i1; i += 4) {
-    const double wi = w[i];
-    const double li = lam[(size_t)i * m + j];
-    const double wl = wi * li;
-    if (kl < pj)
-      at += wl * th[(size_t)i * P + kl];
-    if (kl == 0) {
-      al += wl;
+/* pass 1, draw-major: workgroup (chunk c, part s) streams the packed draw
+ * rows of its draw range (coalesced) and accumulates, per packed
+ * coefficient q (series j(q)), sum w lambda_j theta_q in registers, QU
+ * coefficients per thread per sweep; threads < m also sum w lambda_j and
+ * w log lambda_j.  Partials go to part1s[(c * MOM1_SPLIT + s) * (P + 2m)]. */
+constexpr int MOM1_SPLIT = 16;
+constexpr int MOM1_QU = 8;
+
+__global__ __launch_bounds__(256) void k_mom1(long N, int m, int chunks, const int *qser,
+                                              long P, const double *w, const double *lam,
+                                              const double *theta, double *part1s) {
+  const int c = blockIdx.x / MOM1_SPLIT, sp = blockIdx.x % MOM1_SPLIT;
+  const long c0 = N * c / chunks, c1 = N * (c + 1) / chunks;
+  const long i0 = c0 + (c1 - c0) * sp / MOM1_SPLIT, i1 = c0 + (c1 - c0) * (sp + 1) / MOM1_SPLIT;
+  const int tid = threadIdx.x;
+  double *out = part1s + (size_t)blockIdx.x * (size_t)(P + 2 * m);
+  for (int j = tid; j < m; j += 256) {
+    double al = 0.0, ll = 0.0;
+    for (long i = i0; i < i1; ++i) {
+      const double wi = w[i];
+      const double li = lam[(size_t)i * m + j];
+      al += wi * li;
       ll += wi * log(li);
     }
+    out[P + j] = al;
+    out[P + m + j] = ll;
   }
-  red[dg][kl] = at;
-  if (kl == 0) {
-    red[dg][64] = al;
-    red[dg][65] = ll;
-  }
-  __syncthreads();
-  double *out = part1 + ((size_t)c * m + j) * (max_p + 2);
-  if (dg == 0) {
-    if (kl < max_p)
-      out[kl] = ((red[0][kl] + red[1][kl]) + red[2][kl]) + red[3][kl];
-    if (kl == 0) {
-      out[max_p] = ((red[0][64] + red[1][64]) + red[2][64]) + red[3][64];
-      out[max_p + 1] = ((red[0][65] + red[1][65]) + red[2][65]) + red[3][65];
+  for (long qb = 0; qb < P; qb += 256 * MOM1_QU) {
+    long q[MOM1_QU];
+    int js[MOM1_QU];
+    double acc[MOM1_QU];
+#pragma unroll
+    for (int u = 0; u < MOM1_QU; ++u) {
+      q[u] = qb + tid + 256 * u;
+      js[u] = q[u] < P ? qser[q[u]] : 0;
+      acc[u] = 0.0;
     }
+    for (long i = i0; i < i1; ++i) {
+      const double wi = w[i];
+      const double *row = theta + (size_t)i * P;
+      const double *lrow = lam + (size_t)i * m;
+#pragma unroll
+      for (int u = 0; u < MOM1_QU; ++u) {
+        if (q[u] < P) {
+          const double wl = wi * lrow[js[u]];
+          acc[u] += wl * row[q[u]];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < MOM1_QU; ++u)
+      if (q[u] < P)
+        out[q[u]] = acc[u];
   }
+}
+
+/* pass-1 partials of chunk c in the per-series layout
+ * part1[(c m + j)(max_p + 2) + k]: k < p_j coefficient sums, max_p: sum w
+ * lambda, max_p + 1: sum w log lambda; parts added in a fixed order. */
+__global__ __launch_bounds__(256) void k_mom1_combine(int m, int max_p, const unsigned *p,
+                                                      const int *off, long P,
+                                                      const double *part1s, double *part1) {
+  const int c = blockIdx.y;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= m * (max_p + 2))
+    return;
+  const int j = e / (max_p + 2), k = e - j * (max_p + 2);
+  long src = -1;
+  if (k < (int)p[j])
+    src = off[j] + k;
+  else if (k == max_p)
+    src = P + j;
+  else if (k == max_p + 1)
+    src = P + m + j;
+  double acc = 0.0;
+  if (src >= 0) {
+    const double *ps = part1s + (size_t)c * MOM1_SPLIT * (size_t)(P + 2 * m) + src;
+    for (int s2 = 0; s2 < MOM1_SPLIT; ++s2)
+      acc += ps[(size_t)s2 * (size_t)(P + 2 * m)];
+  }
+  part1[(size_t)c * m * (max_p + 2) + e] = acc;
 }
 
 /* combine pass-1 partials of series j in chunk order */
@@ -405,6 +446,10 @@ __global__ __launch_bounds__(64) void k_vb_final(
 
 } // namespace
 
+size_t vb_part1s_doubles(const Structure &S, long N) {
+  return (size_t)vb_chunks(N) * MOM1_SPLIT * (size_t)(S.P + 2 * S.m);
+}
+
 int vb_chunks(long N) {
   long c = N / 2048;
   if (c < 1)
@@ -423,9 +468,10 @@ void launch_moments(const Structure &S, long N, const double *lam,
                     const double *theta, const double *n_t, const double *s_t,
                     VbBuffers &B, hipStream_t st) {
   const int chunks = B.chunks;
-  hipLaunchKernelGGL(k_mom1, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
-                     S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
-                     B.part1);
+  hipLaunchKernelGGL(k_mom1, dim3(chunks * MOM1_SPLIT), dim3(256), 0, st, N, S.m, chunks,
+                     S.qser, S.P, B.w, lam, theta, B.part1s);
+  hipLaunchKernelGGL(k_mom1_combine, dim3((S.m * (S.max_p + 2) + 255) / 256, chunks), dim3(256),
+                     0, st, S.m, S.max_p, S.p, S.off, S.P, B.part1s, B.part1);
   const int nblk = (S.max_p + 15) / 16;
   switch (nblk) {
   case 1:
