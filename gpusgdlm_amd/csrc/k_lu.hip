@@ -304,6 +304,7 @@ __device__ void diag_block_inverses(const T *Pn, T *Linv, int nq, int wave,
 template <typename T, int NB, int MODE, int RPT, int NT, int NBI>
 __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
   static_assert(NB % NBI == 0 && (MODE == MODE_NATIVE || NBI == NB), "sub-panel width");
+  constexpr int NS = NB / NBI; /* register sub-panels per panel */
   typedef typename Vec4<T>::type V4;
   constexpr int NW = NT / 64;
   constexpr int PS = panel_stride(NB, MODE);
@@ -514,18 +515,18 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
        * factored without interchanges, one barrier per column; between
        * sub-panels the next sub-panel's columns get the previous one's
        * update (phases A and B restricted to those columns, native MFMA) */
-      for (int Ks = K; Ks < K + jb; Ks += NBI) {
-        const int so = Ks - K;              /* sub-panel offset in the panel */
+      /* One loop runs every trailing update of this panel through a single
+       * call site of phases A and B (the kernel stays small enough for the
+       * instruction cache): upd < NS - 1 updates the next sub-panel's
+       * columns with the sub-panels factored so far, upd = NS - 1 is the
+       * full trailing update. */
+      for (int upd = 0; upd < NS; ++upd) {
+        const int so = upd * NBI;            /* columns factored so far */
+        const bool last = upd == NS - 1 || so + NBI >= jb;
+        const int Ks = K + so;               /* first column of this sub-panel */
         const int jbs = min(NBI, K + jb - Ks);
         const int nr = M - Ks;
-        if (so > 0) {
-          diag_block_inverses<T, PS>(Pn, Linv, so >> 4, wave, NW, lane);
-          __syncthreads();
-          phase_a(K, so >> 4, Ks, jbs >> 4);
-          __syncthreads();
-          phase_b(K, so, so >> 4, Ks, jbs >> 4, (M - Ks) >> 4, false);
-          __syncthreads();
-        }
+        {
 
         /* ---- sub-panel rows into registers (16-byte loads) ---- */
         T v[RPT][NBI];
@@ -616,17 +617,24 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
           }
         }
         __syncthreads();
-      }
-      diag_block_inverses<T, PS>(Pn, Linv, nq, wave, NW, lane);
-      __syncthreads();
-
-      const int ntc = (M - K - jb) >> 4;
-      if (ntc > 0) {
-        phase_a(K, nq, K + jb, ntc);
+        }
+        /* trailing update: the next sub-panel's columns, or everything right
+         * of the panel after its last sub-panel */
+        const int kw = last ? jb : so + NBI;          /* factored width */
+        diag_block_inverses<T, PS>(Pn, Linv, kw >> 4, wave, NW, lane);
         __syncthreads();
-        phase_b(K, jb, nq, K + jb, ntc, ntc, MODE == MODE_BF16X3);
+        const int c_first = K + kw;
+        const int ntc = last ? (M - K - jb) >> 4 : min(NBI, jb - kw) >> 4;
+        const int ntr = (M - c_first) >> 4;
+        if (ntc > 0) {
+          phase_a(K, kw >> 4, c_first, ntc);
+          __syncthreads();
+          phase_b(K, kw, kw >> 4, c_first, ntc, ntr, last && MODE == MODE_BF16X3);
+        }
+        __syncthreads();
+        if (last)
+          break;
       }
-      __syncthreads();
     }
 
     /* ---- log|det| = sum log|u_kk| (FP64, fixed reduction order) and the
