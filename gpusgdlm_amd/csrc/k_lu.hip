@@ -139,6 +139,10 @@ struct LuArgs {
   int e_pre_base, e_slot_bytes, e_tab_persist;
   const int *e_slot_init, *e_tab, *e_sg_off, *e_sg_idx, *e_st_off;
   const unsigned *e_st_pairs;
+  /* forecast solves through the prefix (fast kernel, SOLVE) */
+  int e_n2, e_xs_base;
+  const int *e_wave_q, *e_node_of, *e_fwd_ptr, *e_bwd_ptr;
+  const unsigned *e_fwd, *e_bwd;
 };
 
 template <typename T>
@@ -269,6 +273,129 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   __syncthreads();
 }
 
+/* y = (I - Gamma)^{-1} rhs of draw d from the factors the fast kernel
+ * leaves: prefix L and U in the LDS slots, dense L22 / U22 of the Schur
+ * complement in the workspace A (M x M).  x (n1 + M doubles of LDS) is
+ * indexed by prefix position then dense position.  FP64 accumulation;
+ * with a.refine one more pass on the FP64 residual rhs - (I - Gamma) y,
+ * formed with the sparse Gamma in series order. */
+template <typename T, int NT>
+__device__ void prefix_dense_solve(const T *A, int M, char *smem, const double *th,
+                                   const LuArgs &a, long d, int tid) {
+  constexpr unsigned LO = 0xffffu;
+  const int lane = tid & 63, wave = tid >> 6;
+  const T *sl = reinterpret_cast<const T *>(smem + a.e_pre_base);
+  const int *tb = reinterpret_cast<const int *>(smem + a.e_pre_base + a.e_slot_bytes);
+  const int *piv = tb + a.e_o_piv;
+  double *xs = reinterpret_cast<double *>(smem + a.e_xs_base);
+  double *xd = xs + a.e_n1;
+  const int n1 = a.e_n1, n2 = a.e_n2;
+  const double *rhs = a.rhs + (size_t)d * a.m;
+  double *y = a.y + (size_t)d * a.m;
+  for (int pass = 0; pass < (a.refine ? 2 : 1); ++pass) {
+    auto bval = [&](int node) -> double {
+      if (pass == 0)
+        return rhs[node];
+      double acc = rhs[node] - y[node];
+      for (int e = a.g_rowptr[node]; e < a.g_rowptr[node + 1]; ++e)
+        acc += th[a.g_idx[e]] * y[a.g_col[e]];
+      return acc;
+    };
+    /* forward through the prefix, wave by wave */
+    for (int w = 0; w < a.e_waves; ++w) {
+      for (int q = a.e_wave_q[w] + tid; q < a.e_wave_q[w + 1]; q += NT) {
+        double acc = bval(a.e_node_of[q]);
+        for (int e = a.e_fwd_ptr[q]; e < a.e_fwd_ptr[q + 1]; ++e) {
+          const unsigned c = a.e_fwd[e];
+          acc -= (double)sl[c & LO] * xs[c >> 16];
+        }
+        xs[q] = acc;
+      }
+      __syncthreads();
+    }
+    /* right-hand side of the dense part (identity padding: 0) */
+    for (int s2 = tid; s2 < M; s2 += NT) {
+      double acc = 0.0;
+      if (s2 < n2) {
+        acc = bval(a.e_node_of[n1 + s2]);
+        for (int e = a.e_fwd_ptr[n1 + s2]; e < a.e_fwd_ptr[n1 + s2 + 1]; ++e) {
+          const unsigned c = a.e_fwd[e];
+          acc -= (double)sl[c & LO] * xs[c >> 16];
+        }
+      }
+      xd[s2] = acc;
+    }
+    __syncthreads();
+    /* dense: L22 (unit lower) then U22, in blocks of 16 */
+    for (int kb = 0; kb < M / 16; ++kb) {
+      const int b0 = 16 * kb;
+      if (wave == 0 && lane < 16) {
+        double x = xd[b0 + lane];
+        const size_t rowoff = (size_t)(b0 + lane) * M;
+        for (int t = 0; t < 15; ++t) {
+          const double xt = __shfl(x, t, 16);
+          if (lane > t)
+            x -= (double)A[rowoff + b0 + t] * xt;
+        }
+        xd[b0 + lane] = x;
+      }
+      __syncthreads();
+      for (int i = b0 + 16 + tid; i < M; i += NT) {
+        const T *Lr = A + (size_t)i * M + b0;
+        double acc = xd[i];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+          acc -= (double)Lr[t] * xd[b0 + t];
+        xd[i] = acc;
+      }
+      __syncthreads();
+    }
+    for (int kb = M / 16 - 1; kb >= 0; --kb) {
+      const int b0 = 16 * kb;
+      if (wave == 0 && lane < 16) {
+        const size_t rowoff = (size_t)(b0 + lane) * M;
+        double x = xd[b0 + lane];
+        const double dgv = (double)A[rowoff + b0 + lane];
+        for (int t = 15; t >= 0; --t) {
+          if (lane == t)
+            x = x / dgv;
+          const double xt = __shfl(x, t, 16);
+          if (lane < t)
+            x -= (double)A[rowoff + b0 + t] * xt;
+        }
+        xd[b0 + lane] = x;
+      }
+      __syncthreads();
+      for (int i = tid; i < b0; i += NT) {
+        const T *Ur = A + (size_t)i * M + b0;
+        double acc = xd[i];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+          acc -= (double)Ur[t] * xd[b0 + t];
+        xd[i] = acc;
+      }
+      __syncthreads();
+    }
+    /* backward through the prefix, waves in reverse order */
+    for (int w = a.e_waves - 1; w >= 0; --w) {
+      for (int q = a.e_wave_q[w] + tid; q < a.e_wave_q[w + 1]; q += NT) {
+        double acc = xs[q];
+        for (int e = a.e_bwd_ptr[q]; e < a.e_bwd_ptr[q + 1]; ++e) {
+          const unsigned c = a.e_bwd[e];
+          acc -= (double)sl[c & LO] * xs[c >> 16];
+        }
+        xs[q] = acc / (double)sl[piv[q]];
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < n1 + n2; i += NT) {
+      const int node = a.e_node_of[i];
+      y[node] = (pass == 0) ? xs[i] : y[node] + xs[i];
+    }
+    __syncthreads();
+  }
+}
+
 /* Inverses of the nq 16x16 unit-lower diagonal blocks of the panel rows
  * 0..16nq (LDS, stride PS) into Linv[b][i][c]; wave b, lanes 0..15. */
 template <typename T, int PS>
@@ -301,7 +428,7 @@ __device__ void diag_block_inverses(const T *Pn, T *Linv, int nq, int wave,
 /* ========================================================================
  * Fast path: no row interchanges, verified afterwards.
  * ====================================================================== */
-template <typename T, int NB, int MODE, int RPT, int NT, int NBI>
+template <typename T, int NB, int MODE, int RPT, int NT, int NBI, bool SOLVE>
 __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
   static_assert(NB % NBI == 0 && (MODE == MODE_NATIVE || NBI == NB), "sub-panel width");
   constexpr int NS = NB / NBI; /* register sub-panels per panel */
@@ -614,6 +741,14 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
               for (int c = 0; c < NBI; ++c)
                 Pn[pr_ * PS + so + c] = v[rr][c];
             }
+            if constexpr (SOLVE) {
+              /* keep L and U of the panel for the triangular solves */
+              T *dst = A + (size_t)row_of(Ks + r) * M + Ks;
+#pragma unroll
+              for (int c = 0; c < NBI; ++c)
+                if (c < jbs)
+                  dst[c] = v[rr][c];
+            }
           }
         }
         __syncthreads();
@@ -653,22 +788,26 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
       redd[wave] = ls;
     }
     __syncthreads();
+    float mx = redf[0];
+    for (int w = 1; w < NW; ++w)
+      mx = fmaxf(mx, redf[w]);
+    const bool flagged = !(mx <= kMaxMultiplier);
     if (tid == 0) {
-      float mx = redf[0];
       double logdet = redd[0];
-      for (int w = 1; w < NW; ++w) {
-        mx = fmaxf(mx, redf[w]);
+      for (int w = 1; w < NW; ++w)
         logdet += redd[w];
-      }
-      if (!(mx <= kMaxMultiplier)) {
+      if (flagged) {
         const int slot = atomicAdd(a.flag_count, 1);
         a.flag_list[slot] = (int)d;
-        a.logdet[d] = NAN;
-      } else {
-        a.logdet[d] = logdet;
       }
+      if (a.logdet)
+        a.logdet[d] = flagged ? NAN : logdet;
     }
     __syncthreads();
+    if constexpr (SOLVE) {
+      if (!flagged)
+        prefix_dense_solve<T, NT>(A, M, smem, a.theta + (size_t)d * a.P, a, d, tid);
+    }
   }
 }
 
@@ -971,52 +1110,67 @@ int launch_kernel(Kern kern, size_t lds, int grid, int threads, const LuArgs &a,
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 
-template <typename T, int NB, int MODE>
+template <typename T, int NB, int MODE, bool SOLVE>
 int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   /* register sub-panels of 32 columns for 64-wide native panels */
   constexpr int NBI = (MODE == MODE_NATIVE && NB > 32) ? 32 : NB;
-  if (plan.threads == 256) {
-    /* two workgroups per CU; rows per thread up to 2 (M <= 512) */
-    if constexpr (NBI <= 32) {
-      if (a.M <= 256)
-        return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI>, plan.lds, plan.fast_grid, 256, a, st);
-      if constexpr (!(sizeof(T) == 8 && NB == 16)) {
-        if (a.M <= 512)
-          return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256, NBI>, plan.lds, plan.fast_grid, 256, a, st);
+  if constexpr (!SOLVE) {
+    if (plan.threads == 256) {
+      /* two workgroups per CU; rows per thread up to 2 (M <= 512) */
+      if constexpr (NBI <= 32) {
+        if (a.M <= 256)
+          return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI, false>, plan.lds, plan.fast_grid, 256,
+                               a, st);
+        if constexpr (!(sizeof(T) == 8 && NB == 16)) {
+          if (a.M <= 512)
+            return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256, NBI, false>, plan.lds, plan.fast_grid,
+                                 256, a, st);
+        }
       }
+      return SGDLM_ERR_ARG;
     }
-    return SGDLM_ERR_ARG;
   }
+  if (plan.threads != 512)
+    return SGDLM_ERR_ARG;
   if (a.M <= 512)
-    return launch_kernel(k_lu_fast<T, NB, MODE, 1, 512, NBI>, plan.lds, plan.fast_grid, 512, a, st);
+    return launch_kernel(k_lu_fast<T, NB, MODE, 1, 512, NBI, SOLVE>, plan.lds, plan.fast_grid, 512, a, st);
   /* FP64 with 16-wide panels and two rows per thread is never planned (see
    * lu_plan); not instantiating it also avoids a register-coalescer crash of
    * the ROCm 7.2 compiler on that instantiation. */
   if constexpr (!(sizeof(T) == 8 && NB == 16)) {
     if (a.M <= 1024)
-      return launch_kernel(k_lu_fast<T, NB, MODE, 2, 512, NBI>, plan.lds, plan.fast_grid, 512, a, st);
+      return launch_kernel(k_lu_fast<T, NB, MODE, 2, 512, NBI, SOLVE>, plan.lds, plan.fast_grid, 512, a,
+                           st);
   }
   return SGDLM_ERR_ARG;
 }
 
+template <bool SOLVE>
 int dispatch_fast(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   switch (plan.prec) {
   case SGDLM_PREC_FP64:
     if (plan.nb == 64)
-      return fast_rpt<double, 64, MODE_NATIVE>(a, plan, st);
+      return fast_rpt<double, 64, MODE_NATIVE, SOLVE>(a, plan, st);
     if (plan.nb == 32)
-      return fast_rpt<double, 32, MODE_NATIVE>(a, plan, st);
-    return fast_rpt<double, 16, MODE_NATIVE>(a, plan, st);
+      return fast_rpt<double, 32, MODE_NATIVE, SOLVE>(a, plan, st);
+    if constexpr (!SOLVE)
+      return fast_rpt<double, 16, MODE_NATIVE, false>(a, plan, st);
+    return SGDLM_ERR_ARG;
   case SGDLM_PREC_FP32:
     if (plan.nb == 64)
-      return fast_rpt<float, 64, MODE_NATIVE>(a, plan, st);
+      return fast_rpt<float, 64, MODE_NATIVE, SOLVE>(a, plan, st);
     if (plan.nb == 32)
-      return fast_rpt<float, 32, MODE_NATIVE>(a, plan, st);
-    return fast_rpt<float, 16, MODE_NATIVE>(a, plan, st);
+      return fast_rpt<float, 32, MODE_NATIVE, SOLVE>(a, plan, st);
+    if constexpr (!SOLVE)
+      return fast_rpt<float, 16, MODE_NATIVE, false>(a, plan, st);
+    return SGDLM_ERR_ARG;
   case SGDLM_PREC_BF16X3:
-    if (plan.nb == 64)
-      return fast_rpt<float, 64, MODE_BF16X3>(a, plan, st);
-    return fast_rpt<float, 32, MODE_BF16X3>(a, plan, st);
+    if constexpr (!SOLVE) {
+      if (plan.nb == 64)
+        return fast_rpt<float, 64, MODE_BF16X3, false>(a, plan, st);
+      return fast_rpt<float, 32, MODE_BF16X3, false>(a, plan, st);
+    }
+    return SGDLM_ERR_ARG;
   }
   return SGDLM_ERR_ARG;
 }
@@ -1071,7 +1225,22 @@ LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
   a.e_sg_idx = S.e_sg_idx;
   a.e_st_off = S.e_st_off;
   a.e_st_pairs = S.e_st_pairs;
+  a.e_n2 = S.e_n2;
+  a.e_wave_q = S.e_wave_q;
+  a.e_node_of = S.e_node_of;
+  a.e_fwd_ptr = S.e_fwd_ptr;
+  a.e_fwd = S.e_fwd;
+  a.e_bwd_ptr = S.e_bwd_ptr;
+  a.e_bwd = S.e_bwd;
   return a;
+}
+
+void set_fast_layout(LuArgs &a, const Structure &S, const LuPlan &plan) {
+  a.M = plan.fast_M;
+  a.e_pre_base = (int)plan.pre_base;
+  a.e_tab_persist = plan.tab_persist;
+  a.e_slot_bytes = (int)((((size_t)S.e_ns * (plan.prec == SGDLM_PREC_FP64 ? 8 : 4)) + 15) & ~(size_t)15);
+  a.e_xs_base = (int)plan.xs_base;
 }
 
 } // namespace
@@ -1131,6 +1300,26 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
       plan->fast_grid = plan->grid * per_cu;
     }
   }
+  if (solve && prec != SGDLM_PREC_BF16X3) {
+    /* forecast solves: one 512-thread workgroup per CU; the prefix slots and
+     * tables stay in LDS after the dense region, then x (n1 + M2 doubles) */
+    const size_t xs_bytes = (((size_t)S.e_n1 + M2) * sizeof(double) + 15) & ~(size_t)15;
+    for (int i = 0; i < 2 && !plan->nb; ++i) {
+      const int nb = cands[i];
+      if (M2 > 1024)
+        break;
+      const size_t dense = fast_lds_bytes(M2, nb, MODE_NATIVE, ts);
+      if (dense + pre_bytes + xs_bytes <= lds_max) {
+        plan->nb = nb;
+        plan->threads = 512;
+        plan->fast_grid = plan->grid;
+        plan->lds = dense + pre_bytes + xs_bytes;
+        plan->pre_base = dense;
+        plan->tab_persist = 1;
+        plan->xs_base = dense + pre_bytes;
+      }
+    }
+  }
   if (!solve && (want_threads || want_nb) && !plan->nb)
     return SGDLM_ERR_ARG;
   for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
@@ -1166,11 +1355,8 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   }
   if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
     return SGDLM_ERR_DEVICE;
-  a.M = plan.fast_M;
-  a.e_pre_base = (int)plan.pre_base;
-  a.e_tab_persist = plan.tab_persist;
-  a.e_slot_bytes = (int)((((size_t)S.e_ns * (plan.prec == SGDLM_PREC_FP64 ? 8 : 4)) + 15) & ~(size_t)15);
-  int rc = dispatch_fast(a, plan, st);
+  set_fast_layout(a, S, plan);
+  int rc = dispatch_fast<false>(a, plan, st);
   if (rc)
     return rc;
   a.M = S.M;
@@ -1179,11 +1365,24 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
 
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
-                    double *y, int refine, hipStream_t st) {
+                    double *y, int refine, int *flags, hipStream_t st) {
   LuArgs a = base_args(S, N, theta, work);
   a.rhs = rhs;
   a.y = y;
   a.refine = refine;
+  if (plan.nb == 0)
+    return dispatch_pivot<true>(a, plan, st);
+  /* no-interchange fast path with the sparse prefix; draws it flags are
+   * solved by the partial-pivoting kernel */
+  a.flag_count = flags;
+  a.flag_list = flags + 1;
+  if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
+    return SGDLM_ERR_DEVICE;
+  set_fast_layout(a, S, plan);
+  int rc = dispatch_fast<true>(a, plan, st);
+  if (rc)
+    return rc;
+  a.M = S.M;
   return dispatch_pivot<true>(a, plan, st);
 }
 

@@ -255,6 +255,37 @@ void build_elim_plan(int m, const std::vector<GammaEntry> &ents, int max_front,
   for (const auto &t : st)
     pairs += (long long)t.second.size() - 1;
   P.prefix_fma = (long long)contrib.size() + (long long)mult.size() + pairs;
+
+  /* solve tables */
+  std::vector<int> xidx(m, -1); /* series -> x index */
+  for (int q = 0; q < P.n1; ++q)
+    xidx[P.order[q]] = q;
+  P.node_of.assign(P.n1 + P.n2, 0);
+  for (int q = 0; q < P.n1; ++q)
+    P.node_of[q] = P.order[q];
+  for (int i = 0; i < m; ++i)
+    if (P.sep_pos[i] >= 0) {
+      xidx[i] = P.n1 + P.sep_pos[i];
+      P.node_of[P.n1 + P.sep_pos[i]] = i;
+    }
+  P.wave_q.assign(1, 0);
+  for (const auto &wv : waves)
+    P.wave_q.push_back(P.wave_q.back() + (int)wv.size());
+  std::vector<std::vector<uint32_t>> fl(P.n1 + P.n2);
+  P.bwd_ptr.assign(1, 0);
+  for (int q = 0; q < P.n1; ++q) {
+    const int k = P.order[q];
+    for (int i : R[q])
+      fl[xidx[i]].push_back((uint32_t)slot_of.at(key(i, k)) | (uint32_t)q << 16);
+    for (int j : R[q])
+      P.bwd.push_back((uint32_t)slot_of.at(key(k, j)) | (uint32_t)xidx[j] << 16);
+    P.bwd_ptr.push_back((int)P.bwd.size());
+  }
+  P.fwd_ptr.assign(1, 0);
+  for (const auto &v : fl) {
+    P.fwd.insert(P.fwd.end(), v.begin(), v.end());
+    P.fwd_ptr.push_back((int)P.fwd.size());
+  }
 }
 
 } // namespace sgdlm

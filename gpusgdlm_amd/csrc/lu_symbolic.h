@@ -78,6 +78,20 @@ struct ElimPlan {
   std::vector<uint32_t> st_pairs; /* pair (x | y << 16) of block b, round
                                      r, lane l at bcbase[b] + 64 r + l    */
   long long prefix_fma = 0; /* multiply-adds of the prefix + S gather     */
+
+  /* Triangular solves through the prefix (forecasts), with x indexed
+   * [0, n1) = prefix series in elimination order, [n1, n1 + n2) =
+   * separator series in dense order:
+   *   forward:  z_q = b - sum l * z over fwd entries (waves in order), then
+   *             the separator rhs b_s - sum l * z (rows n1.. of fwd_ptr);
+   *   backward: x_q = (z_q - sum u * x) / u_qq (waves in reverse order).
+   * Entries are slot | index << 16. */
+  std::vector<int> wave_q;    /* n_waves + 1: prefix index range per wave  */
+  std::vector<int> node_of;   /* n1 + n2: series of each x index           */
+  std::vector<int> fwd_ptr;   /* n1 + n2 + 1                               */
+  std::vector<uint32_t> fwd;  /* slot(l) | q << 16                         */
+  std::vector<int> bwd_ptr;   /* n1 + 1                                    */
+  std::vector<uint32_t> bwd;  /* slot(u) | x index << 16                   */
 };
 
 /* Builds the plan.  max_front = largest degree a prefix pivot may have

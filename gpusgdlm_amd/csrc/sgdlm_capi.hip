@@ -75,6 +75,7 @@ struct sgdlm_ctx {
   /* simulation memory */
   std::map<std::string, DBuf> sim;
   bool vb_valid = false;
+  bool lu_flags_valid = false; /* lu_flags holds the last LU call's count */
 
   bool timing = false;
   std::map<std::string, std::pair<double, long>> times;
@@ -295,7 +296,10 @@ int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
   int rc;
   if ((rc = upload(c, "e_slot_init", P.slot_init)) || (rc = upload(c, "e_tab", P.tab)) ||
       (rc = upload(c, "e_sg_off", P.sg_off)) || (rc = upload(c, "e_sg_idx", P.sg_idx)) ||
-      (rc = upload(c, "e_st_off", P.st_off)) || (rc = upload(c, "e_st_pairs", P.st_pairs)))
+      (rc = upload(c, "e_st_off", P.st_off)) || (rc = upload(c, "e_st_pairs", P.st_pairs)) ||
+      (rc = upload(c, "e_wave_q", P.wave_q)) || (rc = upload(c, "e_node_of", P.node_of)) ||
+      (rc = upload(c, "e_fwd_ptr", P.fwd_ptr)) || (rc = upload(c, "e_fwd", P.fwd)) ||
+      (rc = upload(c, "e_bwd_ptr", P.bwd_ptr)) || (rc = upload(c, "e_bwd", P.bwd)))
     return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   Structure &S = c->S;
@@ -324,6 +328,12 @@ int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
   S.e_sg_idx = M_<int>(c, "e_sg_idx");
   S.e_st_off = M_<int>(c, "e_st_off");
   S.e_st_pairs = M_<uint32_t>(c, "e_st_pairs");
+  S.e_wave_q = M_<int>(c, "e_wave_q");
+  S.e_node_of = M_<int>(c, "e_node_of");
+  S.e_fwd_ptr = M_<int>(c, "e_fwd_ptr");
+  S.e_fwd = M_<uint32_t>(c, "e_fwd");
+  S.e_bwd_ptr = M_<int>(c, "e_bwd_ptr");
+  S.e_bwd = M_<uint32_t>(c, "e_bwd");
   return SGDLM_OK;
 }
 
@@ -427,6 +437,7 @@ int run_vb(sgdlm_ctx *c) {
   }
   if (rc)
     return fail(c, rc, "LU launch failed: %s", hipGetErrorString(hipGetLastError()));
+  c->lu_flags_valid = true;
   VbBuffers B = vb_buffers(c);
   {
     StageTimer t(c, "weights");
@@ -628,6 +639,7 @@ int sgdlm_init_sim_memory(sgdlm_ctx *c, size_t nsim, size_t nsim_batch) {
   c->nsim = nsim;
   c->nsim_batch = nsim_batch > nsim ? nsim : nsim_batch;
   c->vb_valid = false;
+  c->lu_flags_valid = false;
   c->sim_memory_initialized = true;
   if (c->parents_set)
     return ensure_sim_buffers(c);
@@ -644,6 +656,7 @@ int sgdlm_clear_sim_memory(sgdlm_ctx *c) {
   free_pool(c->sim);
   c->sim_memory_initialized = false;
   c->vb_valid = false;
+  c->lu_flags_valid = false;
   c->nsim = c->nsim_batch = 0;
   return SGDLM_OK;
 }
@@ -998,10 +1011,11 @@ int sgdlm_compute_forecast(sgdlm_ctx *c, double *y, const double *x,
     StageTimer t(c, "forecast_lu");
     rc = launch_lu_solve(c->S, plan, (long)N, SIM<double>(c, "theta"), c->sim["lu_work"].ptr,
                          SIM<double>(c, "rhs"), SIM<double>(c, "yf"),
-                         prec == SGDLM_PREC_FP64 ? 0 : 1, c->stream);
+                         prec == SGDLM_PREC_FP64 ? 0 : 1, SIM<int>(c, "lu_flags"), c->stream);
   }
   if (rc)
     return fail(c, rc, "forecast LU launch failed");
+  c->lu_flags_valid = true;
   c->vb_valid = false;
   if (y && (rc = d2h(c, y, SIM<double>(c, "yf"), N * m * 8)))
     return rc;
@@ -1078,8 +1092,8 @@ int sgdlm_get_lu_fallback_count(const sgdlm_ctx *cc, long *count) {
   sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
   if (!c || !count)
     return SGDLM_ERR_ARG;
-  if (!c->vb_valid)
-    return fail(c, SGDLM_ERR_STATE, "no VB result on the device");
+  if (!c->lu_flags_valid || !c->sim_memory_initialized)
+    return fail(c, SGDLM_ERR_STATE, "no VB or forecast LU has run");
   int n = 0;
   HIPCHK(c, hipMemcpy(&n, SIM<int>(c, "lu_flags"), sizeof(int), hipMemcpyDeviceToHost));
   *count = n;

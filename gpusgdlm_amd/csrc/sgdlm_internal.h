@@ -38,6 +38,8 @@ struct Structure {
   const int *e_slot_init = nullptr, *e_tab = nullptr;
   const int *e_sg_off = nullptr, *e_sg_idx = nullptr, *e_st_off = nullptr;
   const uint32_t *e_st_pairs = nullptr;
+  const int *e_wave_q = nullptr, *e_node_of = nullptr, *e_fwd_ptr = nullptr, *e_bwd_ptr = nullptr;
+  const uint32_t *e_fwd = nullptr, *e_bwd = nullptr;
 };
 
 /* ---- filter (k_filter.hip) ---- */
@@ -81,6 +83,7 @@ struct LuPlan {
   size_t lds = 0;             /* dynamic LDS bytes, fast kernel             */
   size_t pre_base = 0;        /* LDS byte offset of the prefix slots/tables */
   int tab_persist = 0;        /* prefix tables stay in LDS across draws     */
+  size_t xs_base = 0;         /* LDS offset of the solve vector (solves)    */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
   size_t work_bytes = 0;      /* global workspace bytes                      */
 };
@@ -93,9 +96,10 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
                      const double *theta, void *work, double *logdet,
                      int *flags, hipStream_t st);
+/* flags: N + 1 ints of scratch, as for launch_lu_logdet. */
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
-                    double *y, int refine, hipStream_t st);
+                    double *y, int refine, int *flags, hipStream_t st);
 
 /* ---- VB (k_vb.hip) ---- */
 struct VbBuffers {

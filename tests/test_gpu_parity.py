@@ -282,3 +282,20 @@ def test_logdet_lu_tuning_configs(hiplib, prec, cfgs, tol):
     ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th_)
     for lw in out:
         assert np.max(np.abs(lw - ld)) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-8)])
+def test_forecast_m400_fast_path(hiplib, prec, tol):
+    """Forecast solves at m = 400 through the sparse prefix and the
+    no-interchange dense LU (FP32: one FP64-residual refinement) match the
+    oracle's partial-pivoting solves."""
+    mod = random_model(400, 20, T=60, seed=13)
+    g, o = make_pair(mod, prec)
+    filter_steps(g, mod, 1, 50)
+    filter_steps(o, mod, 1, 50)
+    N = 256
+    print(f"forecast m=400 {prec} N={N}", flush=True)
+    yg = g.computeForecast(N, N, mod.F_forecast(50))
+    yo = o.computeForecast(N, N, mod.F_forecast(50))
+    assert g.lu_fallback_count() <= N // 8
+    assert relerr(yg, yo) <= tol

@@ -146,3 +146,26 @@ def test_zero_pivot_is_flagged(chk):
     th = np.array([0.0, 1.0, 0.0, 1.0, 0.0])
     bad, ld, ref, amax, info = run(chk, 3, par, [1, 1, 1], th)
     assert bad == 1
+
+
+@pytest.mark.parametrize("m,kmax,max_front", [(400, 20, 32), (400, 20, 0), (100, 5, 64), (60, 59, 8),
+                                              (1000, 30, 32)])
+def test_solve_through_plan(chk, m, kmax, max_front):
+    """Forecast solves through the plan's prefix tables (forward over the
+    waves, dense LU of S, backward over the waves in reverse) give
+    numpy.linalg.solve's y."""
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    chk.elim_check_solve.restype = C.c_int
+    chk.elim_check_solve.argtypes = [C.c_int, C.c_int, ip, ip, ip, dp, C.c_int, dp, dp]
+    mod = random_model(m, kmax, T=3, seed=12)
+    r, c, ix, P = entries_from_parents(m, mod.parents, mod.n_ext)
+    th = _theta(P, 0.3, 7)
+    b = np.random.default_rng(8).normal(size=m)
+    y = np.zeros(m)
+    chk.elim_check_solve(m, len(r), r.ctypes.data_as(ip), c.ctypes.data_as(ip), ix.ctypes.data_as(ip),
+                         th.ctypes.data_as(dp), max_front, b.ctypes.data_as(dp), y.ctypes.data_as(dp))
+    A = np.eye(m)
+    A[r, c] = -th[ix]
+    ref = np.linalg.solve(A, b)
+    assert np.max(np.abs(y - ref)) <= 1e-11 * max(1.0, np.max(np.abs(ref)))
