@@ -1114,21 +1114,20 @@ template <typename T, int NB, int MODE, bool SOLVE>
 int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   /* register sub-panels of 32 columns for 64-wide native panels */
   constexpr int NBI = (MODE == MODE_NATIVE && NB > 32) ? 32 : NB;
-  if constexpr (!SOLVE) {
-    if (plan.threads == 256) {
-      /* two workgroups per CU; rows per thread up to 2 (M <= 512) */
-      if constexpr (NBI <= 32) {
-        if (a.M <= 256)
-          return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI, false>, plan.lds, plan.fast_grid, 256,
-                               a, st);
-        if constexpr (!(sizeof(T) == 8 && NB == 16)) {
-          if (a.M <= 512)
-            return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256, NBI, false>, plan.lds, plan.fast_grid,
-                                 256, a, st);
-        }
+  if (plan.threads == 256) {
+    /* two workgroups per CU; rows per thread up to 2 (M <= 512); solves
+     * only with 32-wide panels */
+    if constexpr (NBI <= 32 && (!SOLVE || NB == 32)) {
+      if (a.M <= 256)
+        return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI, SOLVE>, plan.lds, plan.fast_grid, 256,
+                             a, st);
+      if constexpr (!(sizeof(T) == 8 && NB == 16)) {
+        if (a.M <= 512)
+          return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256, NBI, SOLVE>, plan.lds, plan.fast_grid,
+                               256, a, st);
       }
-      return SGDLM_ERR_ARG;
     }
+    return SGDLM_ERR_ARG;
   }
   if (plan.threads != 512)
     return SGDLM_ERR_ARG;
@@ -1301,22 +1300,28 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     }
   }
   if (solve && prec != SGDLM_PREC_BF16X3) {
-    /* forecast solves: one 512-thread workgroup per CU; the prefix slots and
-     * tables stay in LDS after the dense region, then x (n1 + M2 doubles) */
+    /* forecast solves: the prefix slots and tables stay in LDS after the
+     * dense region, then x (n1 + M2 doubles); two 256-thread workgroups per
+     * CU (32-wide panels, M2 <= 512) when that fits in half a CU, else one
+     * 512-thread workgroup */
     const size_t xs_bytes = (((size_t)S.e_n1 + M2) * sizeof(double) + 15) & ~(size_t)15;
-    for (int i = 0; i < 2 && !plan->nb; ++i) {
-      const int nb = cands[i];
-      if (M2 > 1024)
-        break;
-      const size_t dense = fast_lds_bytes(M2, nb, MODE_NATIVE, ts);
-      if (dense + pre_bytes + xs_bytes <= lds_max) {
-        plan->nb = nb;
-        plan->threads = 512;
-        plan->fast_grid = plan->grid;
-        plan->lds = dense + pre_bytes + xs_bytes;
-        plan->pre_base = dense;
-        plan->tab_persist = 1;
-        plan->xs_base = dense + pre_bytes;
+    for (int pass = 0; pass < 2 && !plan->nb; ++pass) {
+      const int threads = pass == 0 ? 256 : 512;
+      const int per_cu = pass == 0 ? 2 : 1;
+      for (int i = 0; i < 2 && !plan->nb; ++i) {
+        const int nb = cands[i];
+        if (M2 > 1024 || (threads == 256 && (nb != 32 || M2 > 512)))
+          continue;
+        const size_t dense = fast_lds_bytes(M2, nb, MODE_NATIVE, ts);
+        if (dense + pre_bytes + xs_bytes <= lds_max / per_cu) {
+          plan->nb = nb;
+          plan->threads = threads;
+          plan->fast_grid = plan->grid * per_cu;
+          plan->lds = dense + pre_bytes + xs_bytes;
+          plan->pre_base = dense;
+          plan->tab_persist = 1;
+          plan->xs_base = dense + pre_bytes;
+        }
       }
     }
   }
