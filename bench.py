@@ -88,6 +88,44 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps):
     return out
 
 
+def run_instances(K, N, steps, mmin=20, mmax=400, T_filter=250, seed=20250102):
+    """Many independent models (K instances, m ~ U{mmin..mmax}, k_j ~ U{0..20},
+    T_filter naive filter steps each, N draws, FP32 LU): VB steps of every
+    instance per second, through one batched call per step vs one call per
+    instance."""
+    from gpusgdlm_amd import SGDLM, batch_vb
+    from gpusgdlm_amd.workloads import configure, filter_steps, random_model
+    rng = np.random.default_rng(seed)
+    gs = []
+    for i in range(K):
+        m = int(rng.integers(mmin, mmax + 1))
+        mod = random_model(m, min(20, m - 1), T=T_filter + 2, seed=seed + i)
+        g = SGDLM(strict=True)
+        g.set_precision("fp32")
+        configure(g, mod)
+        filter_steps(g, mod, 1, T_filter)
+        g.computePosterior(mod.y[T_filter], mod.F(T_filter))
+        gs.append(g)
+    batch_vb(gs, N)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch_vb(gs, N)
+    tb = time.perf_counter() - t0
+    for g in gs:
+        g.vb_device(N, N)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for g in gs:
+            g.vb_device(N, N)
+    ts = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    return {"instances": K, "nsim": N, "m_range": [mmin, mmax], "steps": steps,
+            "batched_instance_vb_steps_per_s": K * steps / tb,
+            "sequential_instance_vb_steps_per_s": K * steps / ts,
+            "batched_ms_per_step": 1e3 * tb / steps}
+
+
 def cpu_baseline(mod, T_filter, N_cap, budget_s=12.0):
     """FP64 oracle (OpenMP over draws) on a bounded sample of the same VB step:
     the LU log-determinants dominate (>99% of flops), so the sample is the
@@ -148,6 +186,10 @@ def main():
     ap.add_argument("--precisions", default="fp32,fp64,bf16x3")
     ap.add_argument("--forecast-nsim", type=int, default=100000)
     ap.add_argument("--forecast-steps", type=int, default=10)
+    ap.add_argument("--instances", type=int, default=256,
+                    help="also time K independent models (m ~ U{20..400}, N=1e4) batched vs one call each "
+                         "(0 = skip)")
+    ap.add_argument("--instances-nsim", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (counter calibration)")
@@ -225,6 +267,10 @@ def main():
     }
     if "fp64" in results and head != "fp64":
         line["speedup_vs_fp64"] = results["fp64"]["ms_per_step"] / r["ms_per_step"]
+    if a.instances:
+        log(f"[instances] {a.instances} models, N={a.instances_nsim}")
+        line["many_instances"] = run_instances(a.instances, a.instances_nsim, 3)
+        log(f"[instances] {line['many_instances']}")
     if not a.no_cpu:
         cb = cpu_baseline(mod, a.filter_steps, N)
         cpu_steps = 1.0 / (cb["dt"] * N / cb["n"])

@@ -910,6 +910,33 @@ int sgdlm_compute_vb_posterior(sgdlm_ctx *c, double *mean_m, double *mean_C,
   return SGDLM_OK;
 }
 
+int sgdlm_batch_compute_vb_posterior(sgdlm_ctx *const *ctxs, size_t count, int *status) {
+  if (!ctxs && count)
+    return SGDLM_ERR_ARG;
+  int first = SGDLM_OK;
+  std::vector<int> rcs(count, SGDLM_OK);
+  /* enqueue every context's stages on its own stream, then wait once each */
+  for (size_t i = 0; i < count; ++i) {
+    sgdlm_ctx *c = ctxs[i];
+    int rc = c ? SGDLM_OK : SGDLM_ERR_ARG;
+    if (!rc) {
+      if (!(rc = check_init(c)) && !(rc = check_parents(c)) && !(rc = check_sim(c)) &&
+          !(rc = check_prior(c, false)))
+        rc = run_vb(c);
+    }
+    rcs[i] = rc;
+  }
+  for (size_t i = 0; i < count; ++i) {
+    if (!rcs[i])
+      rcs[i] = sync_and_collect(ctxs[i]);
+    if (status)
+      status[i] = rcs[i];
+    if (rcs[i] && !first)
+      first = rcs[i];
+  }
+  return first;
+}
+
 int sgdlm_get_vb_result(const sgdlm_ctx *cc, double *mean_m, double *mean_C,
                         double *mean_n, double *mean_s, double *w,
                         double *log_w, double *ess, double *log_sum_det) {

@@ -385,3 +385,23 @@ class SGDLM:
                    C.byref(fma))
         return {"max_front": mf.value, "n_prefix": n1.value, "n_dense": n2.value, "waves": nw.value,
                 "prefix_fma": fma.value}
+
+
+def batch_vb(models, nsim, nsim_batch=None):
+    """computeVBPosterior (results left on the device) for a list of SGDLM
+    objects in one batched call: the GPU overlaps the instances.  Returns the
+    list of per-model status codes; results via get_vb_result()."""
+    lib = nat.load()
+    nb = int(nsim_batch or nsim)
+    for g in models:
+        if not g._ensure_sim(int(nsim), nb):
+            raise SGDLMError(nat.SGDLM_ERR_STATE, "could not initialise simulation memory")
+    arr = (C.c_void_p * len(models))(*[g._h.value for g in models])
+    st = (C.c_int * len(models))()
+    rc = lib.sgdlm_batch_compute_vb_posterior(arr, C.c_size_t(len(models)), st)
+    codes = list(st)
+    if rc != nat.SGDLM_OK and any(g.strict for g in models):
+        bad = [i for i, c in enumerate(codes) if c]
+        raise SGDLMError(rc, f"batched VB failed for models {bad[:8]}")
+    return codes
+

@@ -146,6 +146,17 @@ int sgdlm_get_vb_result(const sgdlm_ctx *ctx, double *mean_m_t,
                         double *mean_C_t, double *mean_n_t, double *mean_s_t,
                         double *IS_weights, double *log_weights, double *ess,
                         double *log_sum_det);
+/* Batched VB for many independent models (one context each, all on this
+ * GPU): runs sgdlm_compute_vb_posterior with device-resident results on
+ * ctxs[0..count), enqueueing every context's stages on its own stream before
+ * waiting for any, so the device overlaps small instances; fetch results
+ * with sgdlm_get_vb_result.  status (optional, count entries) receives each
+ * context's status; the return value is the first non-OK status.  Contexts
+ * must not be used concurrently from other threads during the call.  Serves
+ * the reference's many-instance use (one HostWrapper per model) without a
+ * host round trip per model. */
+int sgdlm_batch_compute_vb_posterior(sgdlm_ctx *const *ctxs, size_t count,
+                                     int *status);
 /* Per-stage device timing with HIP events recorded on the context's stream
  * around each launch (enable = 1 resets the counters).  stage: "lu" (VB
  * log-det LU), "draw", "weights", "moments", "forecast_lu", "filter". */

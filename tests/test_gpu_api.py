@@ -109,3 +109,43 @@ def test_evolution_matrix_prior_and_forecast(hiplib):
     ro1 = o.computeEvoForecast(500, 500, Fk, G[..., 0])
     assert np.max(np.abs(rg1 - ro1)) / np.max(np.abs(ro1)) <= 1e-9
     g.close()
+
+
+def test_batched_vb_equals_sequential(hiplib):
+    """sgdlm_batch_compute_vb_posterior over independent models gives, bit
+    for bit, what one computeVBPosterior per model gives (same seeds and
+    epochs; one stream per context)."""
+    from gpusgdlm_amd import batch_vb
+    from gpusgdlm_amd.workloads import random_model
+    mods = [random_model(m, min(k, m - 1), T=30, seed=100 + i)
+            for i, (m, k) in enumerate([(20, 4), (60, 8), (150, 12), (33, 5), (90, 20), (7, 3)])]
+    N = 2000
+
+    def build():
+        gs = []
+        for mod in mods:
+            g = SGDLM(strict=True)
+            g.set_rng_state(77, 0)
+            g.set_precision("fp32")
+            configure(g, mod)
+            filter_steps(g, mod, 1, 20)
+            g.computePosterior(mod.y[20], mod.F(20))
+            gs.append(g)
+        return gs
+
+    ga, gb = build(), build()
+    print(f"batched VB over {len(mods)} models, N={N}", flush=True)
+    codes = batch_vb(ga, N)
+    assert codes == [0] * len(mods)
+    codes = batch_vb(ga, N)            # a second batched step (epochs advance alike)
+    assert codes == [0] * len(mods)
+    for g in gb:
+        g.vb_device(N, N)
+        g.vb_device(N, N)
+    for x, y in zip(ga, gb):
+        rx, ry = x.get_vb_result(), y.get_vb_result()
+        for k in ("m", "C", "n", "s", "IS_weights"):
+            assert np.array_equal(rx[k], ry[k]), k
+        assert x.get_rng_state() == y.get_rng_state()
+    for g in ga + gb:
+        g.close()
