@@ -1179,14 +1179,14 @@ int dispatch_pivot(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   /* BF16x3 falls back to FP32 partial pivoting */
   if (plan.prec == SGDLM_PREC_FP64) {
     if (plan.pivot_nb == 32)
-      return launch_kernel(k_lu_pivot<double, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
-    return launch_kernel(k_lu_pivot<double, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+      return launch_kernel(k_lu_pivot<double, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
+    return launch_kernel(k_lu_pivot<double, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
   }
   if (plan.pivot_nb == 64)
-    return launch_kernel(k_lu_pivot<float, 64, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+    return launch_kernel(k_lu_pivot<float, 64, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
   if (plan.pivot_nb == 32)
-    return launch_kernel(k_lu_pivot<float, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
-  return launch_kernel(k_lu_pivot<float, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.grid, 256, a, st);
+    return launch_kernel(k_lu_pivot<float, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
+  return launch_kernel(k_lu_pivot<float, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
 }
 
 LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
@@ -1327,19 +1327,27 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   }
   if (!solve && (want_threads || want_nb) && !plan->nb)
     return SGDLM_ERR_ARG;
-  for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
-    const int nb = cands[i];
-    if (prec == SGDLM_PREC_FP64 && nb > 32)
-      continue;
-    const size_t lds = pivot_lds_bytes(M, nb, ts, solve != 0);
-    if (lds <= lds_max) {
-      plan->pivot_nb = nb;
-      plan->pivot_lds = lds;
+  /* partial pivoting: two workgroups per CU when a 32- or 16-wide panel
+   * fits in half a CU (its per-column pivot searches are latency-bound),
+   * else one with the widest panel that fits */
+  plan->pivot_grid = plan->grid;
+  for (int per_cu = 2; per_cu >= 1 && !plan->pivot_nb; --per_cu) {
+    for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
+      const int nb = cands[i];
+      if ((prec == SGDLM_PREC_FP64 || per_cu == 2) && nb > 32)
+        continue;
+      const size_t lds = pivot_lds_bytes(M, nb, ts, solve != 0);
+      if (lds <= lds_max / per_cu) {
+        plan->pivot_nb = nb;
+        plan->pivot_lds = lds;
+        plan->pivot_grid = plan->grid * per_cu;
+      }
     }
   }
   if (plan->pivot_nb == 0)
     return SGDLM_ERR_ARG;
-  plan->work_bytes = (size_t)std::max(plan->grid, plan->fast_grid) * M * M * ts;
+  plan->work_bytes =
+      (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * M * M * ts;
   return SGDLM_OK;
 }
 

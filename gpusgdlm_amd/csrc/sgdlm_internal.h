@@ -77,7 +77,8 @@ struct LuPlan {
   int fast_M = 0;             /* dense dimension of the fast kernel (the
                                  Schur complement after the sparse prefix)  */
   int pivot_nb = 0;           /* panel width, partial-pivoting kernel       */
-  int grid = 0;               /* persistent workgroups (pivoting kernel)    */
+  int grid = 0;               /* CUs (persistent workgroups per pass)       */
+  int pivot_grid = 0;         /* persistent workgroups, pivoting kernel     */
   int threads = 512;          /* workgroup size of the fast kernel          */
   int fast_grid = 0;          /* persistent workgroups of the fast kernel   */
   size_t lds = 0;             /* dynamic LDS bytes, fast kernel             */
