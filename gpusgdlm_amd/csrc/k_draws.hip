@@ -9,11 +9,12 @@
  * batchedComputeNuPlusMu (kernel_functions.cuh:1155-1178) and
  * evolve_lambdas_and_thetas (SGDLM.cu:679-762).
  *
- * Thread mapping: blockIdx.y = series j, consecutive threads = consecutive
+ * Thread mapping: blockIdx.x = series j, consecutive threads = consecutive
  * draws of that series, so a wavefront shares p_j, n_j, s_j and reads the
  * same Cholesky factor (broadcast loads) with no loop divergence except the
  * gamma sampler's rare second attempt.  Per-thread accumulators live in LDS
- * with the thread index as the fastest dimension (bank-conflict free).
+ * with the thread index as the fastest dimension (odd row stride); k_draw
+ * then writes each draw's theta segment with consecutive threads.
  * Random numbers come from counter-based Philox, so every (draw, series)
  * value is independent of launch shape and identical to the host oracle.
  */
@@ -26,15 +27,61 @@ namespace sgdlm {
 namespace {
 
 constexpr int TB = 128;
+constexpr int TS = TB + 1; /* LDS stride of per-thread accumulators (odd:
+                              row and column reads spread over the banks) */
+constexpr int MAX_GRID_Y = 65535;
 
-__global__ __launch_bounds__(TB) void k_cholesky(int m, int max_p,
-                                                 const unsigned *p,
+/* Grids of the per-(draw, series) kernels: blockIdx.x = series j (fastest,
+ * so the workgroups in flight write neighbouring series of the same draws
+ * and complete whole cache lines of the draw-major theta rows), blockIdx.y =
+ * blocks of TB draws, strided when there are more than MAX_GRID_Y. */
+dim3 grid_nm(long N, int m) {
+  const long nblk = (N + TB - 1) / TB;
+  return dim3((unsigned)m, (unsigned)(nblk < MAX_GRID_Y ? nblk : MAX_GRID_Y));
+}
+
+/* One workgroup (one wavefront) per series: the right-looking Cholesky of
+ * sg_cholesky_upper with one element per lane and a barrier per column.
+ * Every element sees the same operations on the same operands in the same
+ * order as the serial loop, so U is bit-identical to it. */
+__global__ __launch_bounds__(64) void k_cholesky(int m, int max_p, const unsigned *p,
                                                  const double *C, double *U) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < m) {
-    const size_t o = (size_t)j * max_p * max_p;
-    sg_cholesky_upper((int)p[j], max_p, max_p, C + o, U + o);
+  extern __shared__ double a[]; /* [max_p][max_p], a[k * max_p + n] */
+  const int j = blockIdx.x, t = threadIdx.x;
+  const int pj = (int)p[j], ld = max_p, nn = max_p * max_p;
+  const size_t o = (size_t)j * nn;
+  for (int e = t; e < nn; e += 64) {
+    const int k = e / ld, n = e - k * ld;
+    a[e] = (n < pj && k >= n && k < pj) ? C[o + e] : 0.0;
   }
+  __syncthreads();
+  for (int n = 0; n < pj; ++n) {
+    const double dn = a[n * ld + n];
+    __syncthreads();
+    if (!(dn > 0.0)) {
+      /* zero or invalid pivot of a semi-definite block: the column is zero */
+      for (int k = n + t; k < pj; k += 64)
+        a[k * ld + n] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    const double diag = __builtin_sqrt(dn);
+    const double inv = 1.0 / diag;
+    if (t == 0)
+      a[n * ld + n] = diag;
+    for (int k = n + 1 + t; k < pj; k += 64)
+      a[k * ld + n] = a[k * ld + n] * inv;
+    __syncthreads();
+    const int nt = pj - n - 1;
+    for (int e = t; e < nt * nt; e += 64) {
+      const int k = n + 1 + e / nt, l = n + 1 + e % nt;
+      if (l >= k)
+        a[l * ld + k] = a[l * ld + k] - a[k * ld + n] * a[l * ld + n];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nn; e += 64)
+    U[o + e] = a[e];
 }
 
 __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
@@ -44,20 +91,30 @@ __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
                                              const double *s_t, uint64_t seed,
                                              uint32_t epoch, double *lam,
                                              double *theta) {
-  extern __shared__ double sacc[]; /* [max_p][TB] */
-  const int j = blockIdx.y;
-  const long i = (long)blockIdx.x * TB + threadIdx.x;
-  if (i >= N)
-    return;
+  extern __shared__ double sacc[]; /* [max_p][TS] */
+  const int j = blockIdx.x;
   const int pj = (int)p[j];
-  double *acc = sacc + threadIdx.x;
-  const double l = sg_draw_lambda_theta(
-      pj, max_p, m_t + (size_t)j * max_p, U + (size_t)j * max_p * max_p, n_t[j],
-      s_t[j], seed, epoch, (uint32_t)i, (uint32_t)j, acc, TB);
-  lam[(size_t)i * m + j] = l;
-  double *out = theta + (size_t)i * P + off[j];
-  for (int k = 0; k < pj; ++k)
-    out[k] = acc[k * TB];
+  const long nblk = (N + TB - 1) / TB;
+  for (long ib = blockIdx.y; ib < nblk; ib += gridDim.y) {
+    const long i0 = ib * TB;
+    const int nb = (int)(N - i0 < TB ? N - i0 : TB);
+    const long i = i0 + threadIdx.x;
+    if ((int)threadIdx.x < nb) {
+      const double l = sg_draw_lambda_theta(
+          pj, max_p, m_t + (size_t)j * max_p, U + (size_t)j * max_p * max_p, n_t[j],
+          s_t[j], seed, epoch, (uint32_t)i, (uint32_t)j, sacc + threadIdx.x, TS);
+      lam[(size_t)i * m + j] = l;
+    }
+    __syncthreads();
+    /* the block's segments theta[i][off_j .. off_j + p_j): consecutive
+     * threads write consecutive coefficients */
+    double *base = theta + (size_t)i0 * P + off[j];
+    for (int e = threadIdx.x; e < nb * pj; e += TB) {
+      const int il = e / pj, k = e - il * pj;
+      base[(size_t)il * P + k] = sacc[k * TS + il];
+    }
+    __syncthreads();
+  }
 }
 
 /* rhs_ij = nu_ij / sqrt(lambda_ij) + sum_{k: sp_jk external} x_jk theta_ijk
@@ -67,40 +124,33 @@ __global__ __launch_bounds__(TB) void k_forecast_rhs(
     const int *off, long P, const double *x, const double *m_t,
     const double *theta, const double *lam, uint64_t seed, uint32_t epoch,
     double *rhs) {
-  const int j = blockIdx.y;
-  const long i = (long)blockIdx.x * TB + threadIdx.x;
-  if (i >= N)
-    return;
+  const int j = blockIdx.x;
   const int pj = (int)p[j];
   const unsigned mm = (unsigned)m * (unsigned)m;
-  const double *th = theta + (size_t)i * P + off[j];
-  double nu = sg_normal_k(seed, epoch, SG_PURPOSE_NU, (uint32_t)i, (uint32_t)j, 0u);
-  nu = nu / __builtin_sqrt(lam[(size_t)i * m + j]);
-  for (int k = 0; k < max_p; ++k) {
-    if (sp[(size_t)j * max_p + k] >= mm) {
-      const double t = (k < pj) ? th[k] : m_t[(size_t)j * max_p + k];
-      nu = nu + x[(size_t)j * max_p + k] * t;
+  for (long i = (long)blockIdx.y * TB + threadIdx.x; i < N; i += (long)gridDim.y * TB) {
+    const double *th = theta + (size_t)i * P + off[j];
+    double nu = sg_normal_k(seed, epoch, SG_PURPOSE_NU, (uint32_t)i, (uint32_t)j, 0u);
+    nu = nu / __builtin_sqrt(lam[(size_t)i * m + j]);
+    for (int k = 0; k < max_p; ++k) {
+      if (sp[(size_t)j * max_p + k] >= mm) {
+        const double t = (k < pj) ? th[k] : m_t[(size_t)j * max_p + k];
+        nu = nu + x[(size_t)j * max_p + k] * t;
+      }
     }
+    rhs[(size_t)i * m + j] = nu;
   }
-  rhs[(size_t)i * m + j] = nu;
 }
 
-/* lambda *= eta (beta draw), omega_k = z_k / sqrt(c lambda) for k < p_j,
- * theta = G theta (optional), theta += chol(W)^T omega
- * (SGDLM.cu:679-762 with the intended scale_omegas indexing). */
-__global__ __launch_bounds__(TB) void k_evolve(
-    long N, int m, int max_p, const unsigned *p, const int *off, long P,
-    const double *U_W, const double *r_t, const double *c_t,
-    const double *beta, const double *G, const double *m_t, uint64_t seed,
-    uint32_t epoch, double *lam, double *theta) {
-  extern __shared__ double sbuf[]; /* omega [max_p][TB], old theta [max_p][TB] */
-  const int j = blockIdx.y;
-  const long i = (long)blockIdx.x * TB + threadIdx.x;
-  if (i >= N)
-    return;
+/* one (draw, series) of k_evolve: lambda *= eta (beta draw),
+ * omega_k = z_k / sqrt(c lambda) for k < p_j, theta = G theta (optional),
+ * theta += chol(W)^T omega (SGDLM.cu:679-762 with the intended
+ * scale_omegas indexing). */
+__device__ void evolve_one(long i, int j, int m, int max_p, const unsigned *p, const int *off,
+                           long P, const double *U_W, const double *r_t, const double *c_t,
+                           const double *beta, const double *G, const double *m_t,
+                           uint64_t seed, uint32_t epoch, double *lam, double *theta,
+                           double *om, double *ot) {
   const int pj = (int)p[j];
-  double *om = sbuf + threadIdx.x;
-  double *ot = sbuf + (size_t)max_p * TB + threadIdx.x;
   double *th = theta + (size_t)i * P + off[j];
   const double *U = U_W + (size_t)j * max_p * max_p;
   double *lp = lam + (size_t)i * m + j;
@@ -139,38 +189,47 @@ __global__ __launch_bounds__(TB) void k_evolve(
   }
 }
 
+__global__ __launch_bounds__(TB) void k_evolve(
+    long N, int m, int max_p, const unsigned *p, const int *off, long P,
+    const double *U_W, const double *r_t, const double *c_t,
+    const double *beta, const double *G, const double *m_t, uint64_t seed,
+    uint32_t epoch, double *lam, double *theta) {
+  extern __shared__ double sbuf[]; /* omega [max_p][TB], old theta [max_p][TB] */
+  const int j = blockIdx.x;
+  for (long i = (long)blockIdx.y * TB + threadIdx.x; i < N; i += (long)gridDim.y * TB)
+    evolve_one(i, j, m, max_p, p, off, P, U_W, r_t, c_t, beta, G, m_t, seed, epoch, lam, theta,
+               sbuf + threadIdx.x, sbuf + (size_t)max_p * TB + threadIdx.x);
+}
+
 __global__ __launch_bounds__(TB) void k_unpack_theta(long N, int m, int max_p,
                                                      const unsigned *p,
                                                      const int *off, long P,
                                                      const double *theta,
                                                      const double *m_t,
                                                      double *out) {
-  const int j = blockIdx.y;
-  const long i = (long)blockIdx.x * TB + threadIdx.x;
-  if (i >= N)
-    return;
+  const int j = blockIdx.x;
   const int pj = (int)p[j];
-  const double *th = theta + (size_t)i * P + off[j];
-  double *o = out + ((size_t)i * m + j) * max_p;
-  for (int k = 0; k < max_p; ++k)
-    o[k] = (k < pj) ? th[k] : m_t[(size_t)j * max_p + k];
+  for (long i = (long)blockIdx.y * TB + threadIdx.x; i < N; i += (long)gridDim.y * TB) {
+    const double *th = theta + (size_t)i * P + off[j];
+    double *o = out + ((size_t)i * m + j) * max_p;
+    for (int k = 0; k < max_p; ++k)
+      o[k] = (k < pj) ? th[k] : m_t[(size_t)j * max_p + k];
+  }
 }
-
-dim3 grid_nm(long N, int m) { return dim3((unsigned)((N + TB - 1) / TB), (unsigned)m); }
 
 } // namespace
 
 void launch_cholesky(const Structure &S, const double *C, double *U,
                      hipStream_t st) {
-  hipLaunchKernelGGL(k_cholesky, dim3((S.m + TB - 1) / TB), dim3(TB), 0, st,
-                     S.m, S.max_p, S.p, C, U);
+  hipLaunchKernelGGL(k_cholesky, dim3(S.m), dim3(64),
+                     (size_t)S.max_p * S.max_p * sizeof(double), st, S.m, S.max_p, S.p, C, U);
 }
 
 void launch_draw(const Structure &S, long N, const double *m_t,
                  const double *U, const double *n_t, const double *s_t,
                  uint64_t seed, uint32_t epoch, double *lam, double *theta,
                  hipStream_t st) {
-  const size_t lds = (size_t)S.max_p * TB * sizeof(double);
+  const size_t lds = (size_t)S.max_p * TS * sizeof(double);
   hipLaunchKernelGGL(k_draw, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
                      S.max_p, S.p, S.off, S.P, m_t, U, n_t, s_t, seed, epoch,
                      lam, theta);

@@ -54,6 +54,7 @@
 #include <algorithm>
 #include <math.h>
 
+#include "lu_symbolic.h"
 #include "sgdlm_internal.h"
 
 namespace sgdlm {
@@ -183,13 +184,44 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   const int lane = tid & 63, wave = tid >> 6;
   T *sl = reinterpret_cast<T *>(smem + a.e_pre_base);
   int *tb = reinterpret_cast<int *>(smem + a.e_pre_base + a.e_slot_bytes);
+  /* the global reads below are issued GB at a time per thread: one
+   * round trip per batch instead of one per element */
+  constexpr int GB = 8;
   if (load_tab) {
-    for (int w = tid; w < a.e_tab_words; w += nthreads)
-      tb[w] = a.e_tab[w];
+    for (int w0 = tid; w0 < a.e_tab_words; w0 += GB * nthreads) {
+      int x[GB];
+#pragma unroll
+      for (int u = 0; u < GB; ++u) {
+        const int w = w0 + u * nthreads;
+        x[u] = w < a.e_tab_words ? a.e_tab[w] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < GB; ++u) {
+        const int w = w0 + u * nthreads;
+        if (w < a.e_tab_words)
+          tb[w] = x[u];
+      }
+    }
   }
-  for (int s = tid; s < a.e_ns; s += nthreads) {
-    const int v = a.e_slot_init[s];
-    sl[s] = v >= 0 ? (T)(-th[v]) : (v == -2 ? (T)1 : (v == -3 ? (T)-1 : (T)0));
+  for (int s0 = tid; s0 < a.e_ns; s0 += GB * nthreads) {
+    int code[GB];
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int s = s0 + u * nthreads;
+      code[u] = s < a.e_ns ? a.e_slot_init[s] : SLOT_ZERO;
+    }
+    T x[GB];
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int v = code[u];
+      x[u] = v >= 0 ? (T)(-th[v]) : (v == SLOT_ONE ? (T)1 : (v == SLOT_MINUS_ONE ? (T)-1 : (T)0));
+    }
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int s = s0 + u * nthreads;
+      if (s < a.e_ns)
+        sl[s] = x[u];
+    }
   }
   const size_t nvec = (size_t)M * M * sizeof(T) / 16;
   float4 *A4 = reinterpret_cast<float4 *>(A);
@@ -237,8 +269,23 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   }
   for (int r = tid; r < M; r += nthreads)
     A[(size_t)r * M + r] = (T)1;
-  for (int e = tid; e < a.e_nsg; e += nthreads)
-    A[a.e_sg_off[e]] = (T)(-th[a.e_sg_idx[e]]);
+  for (int e0 = tid; e0 < a.e_nsg; e0 += GB * nthreads) {
+    int off[GB], ix[GB];
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      const int e = e0 + u * nthreads;
+      off[u] = e < a.e_nsg ? a.e_sg_off[e] : -1;
+      ix[u] = e < a.e_nsg ? a.e_sg_idx[e] : 0;
+    }
+    T x[GB];
+#pragma unroll
+    for (int u = 0; u < GB; ++u)
+      x[u] = off[u] >= 0 ? (T)(-th[ix[u]]) : (T)0;
+#pragma unroll
+    for (int u = 0; u < GB; ++u)
+      if (off[u] >= 0)
+        A[off[u]] = x[u];
+  }
   __syncthreads();
   const int *brounds = tb + a.e_o_brounds, *bcbase = tb + a.e_o_bcbase;
   const unsigned *pairs = a.e_st_pairs;

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
  * coefficient q (series j(q)), sum w lambda_j theta_q in registers, QU
  * coefficients per thread per sweep; threads < m also sum w lambda_j and
  * w log lambda_j.  Partials go to part1s[(c * MOM1_SPLIT + s) * (P + 2m)]. */
-constexpr int MOM1_SPLIT = 16;
+constexpr int MOM1_SPLIT = 64;
 constexpr int MOM1_QU = 8;
 
 __global__ __launch_bounds__(256) void k_mom1(long N, int m, int chunks, const int *qser,
@@ -152,6 +152,7 @@ __global__ __launch_bounds__(256) void k_mom1(long N, int m, int chunks, const i
   double *out = part1s + (size_t)blockIdx.x * (size_t)(P + 2 * m);
   for (int j = tid; j < m; j += 256) {
     double al = 0.0, ll = 0.0;
+#pragma unroll 4
     for (long i = i0; i < i1; ++i) {
       const double wi = w[i];
       const double li = lam[(size_t)i * m + j];
@@ -171,6 +172,7 @@ __global__ __launch_bounds__(256) void k_mom1(long N, int m, int chunks, const i
       js[u] = q[u] < P ? qser[q[u]] : 0;
       acc[u] = 0.0;
     }
+#pragma unroll 4
     for (long i = i0; i < i1; ++i) {
       const double wi = w[i];
       const double *row = theta + (size_t)i * P;
@@ -265,6 +267,7 @@ __global__ __launch_bounds__(256) void k_mom2(long N, int m, int max_p,
     acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
   const double *th = theta + off[j];
   const long nsteps = (i1 - i0 + 3) / 4;
+#pragma unroll 4
   for (long s = wave; s < nsteps; s += 4) {
     const long i = i0 + 4 * s + g;
     double x[NBLK];
