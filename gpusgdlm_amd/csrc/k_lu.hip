@@ -516,38 +516,58 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     /* ---- phase A: U12 = L11^{-1} A12 for the column tiles
      * [c_first, c_first + 16 ntc), L11 = the nqp 16-row blocks of the panel
      * (Pn rows 0..16 nqp, Linv), written in place of A12 ---- */
+    /* Two column tiles per wavefront at a time (h = 0, 1): their block
+     * forward substitutions are independent MFMA chains that share the A
+     * operands, so the second hides the first's MFMA latency. */
     auto phase_a = [&](int Kp, int nqp, int c_first, int ntc) {
-      for (int ct = wave; ct < ntc; ct += NW) {
-        const int c0 = c_first + 16 * ct;
-        V4 u[NQ];
+      for (int ct = wave; ct < ntc; ct += 2 * NW) {
+        const bool two = ct + NW < ntc;
+        V4 u[2][NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          V4 acc = {0, 0, 0, 0};
-          if (q < nqp) {
+        for (int h = 0; h < 2; ++h) {
+          const int c0 = c_first + 16 * (ct + h * NW);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-              acc[r] = A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+          for (int q = 0; q < NQ; ++q) {
+            V4 acc = {0, 0, 0, 0};
+            if (q < nqp && (h == 0 || two)) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                acc[r] = A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c0 + l16];
+            }
+            u[h][q] = acc;
           }
-          u[q] = acc;
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           if (q < nqp) {
-            V4 acc = u[q];
+            V4 acc0 = u[0][q], acc1 = u[1][q];
 #pragma unroll
             for (int aa = 0; aa < NQ; ++aa)
               if (aa < q)
 #pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  const T av = -Pn[(16 * q + l16) * PS + 16 * aa + rho<T>(g, r)];
+                  acc0 = mfma16(av, u[0][aa][r], acc0);
+                  acc1 = mfma16(av, u[1][aa][r], acc1);
+                }
+            V4 uq0 = {0, 0, 0, 0}, uq1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const T lv = Linv[q * 256 + l16 * 16 + rho<T>(g, r)];
+              uq0 = mfma16(lv, acc0[r], uq0);
+              uq1 = mfma16(lv, acc1[r], uq1);
+            }
+            u[0][q] = uq0;
+            u[1][q] = uq1;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if (h == 0 || two) {
+                const int c0 = c_first + 16 * (ct + h * NW);
+#pragma unroll
                 for (int r = 0; r < 4; ++r)
-                  acc = mfma16(-Pn[(16 * q + l16) * PS + 16 * aa + rho<T>(g, r)], u[aa][r], acc);
-            V4 uq = {0, 0, 0, 0};
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              uq = mfma16(Linv[q * 256 + l16 * 16 + rho<T>(g, r)], acc[r], uq);
-            u[q] = uq;
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[q][r];
+                  A[(size_t)row_of(Kp + 16 * q + rho<T>(g, r)) * M + c0 + l16] = u[h][q][r];
+              }
+            }
           }
         }
       }
@@ -742,11 +762,20 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
               bad = 1;
               dj = (T)1;
             }
+            /* FP32: multipliers through the hardware reciprocal (1 ulp),
+             * off the IEEE division's longer dependent sequence */
+            T rdj = (T)0;
+            if constexpr (sizeof(T) == 4)
+              rdj = __builtin_amdgcn_rcpf(dj);
 #pragma unroll
             for (int rr = 0; rr < RPT; ++rr) {
               const int r = tid + NT * rr;
               if (r > j && r < nr) {
-                const T l = v[rr][j] / dj;
+                T l;
+                if constexpr (sizeof(T) == 4)
+                  l = v[rr][j] * rdj;
+                else
+                  l = v[rr][j] / dj;
                 v[rr][j] = l;
                 amax = fmax(amax, fabs(l));
 #pragma unroll
