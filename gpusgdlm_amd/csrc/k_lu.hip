@@ -309,9 +309,15 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
 #pragma unroll
       for (int r = 0; r < RR; ++r)
         acc = fma(-sl[pv[u][r] & LO], sl[pv[u][r] >> 16], acc);
-      for (int r = RR; r < R[u]; ++r) {
-        const unsigned c = pairs[(size_t)cb[u] + 64 * r + lane];
-        acc = fma(-sl[c & LO], sl[c >> 16], acc);
+      /* longer targets: further rounds RR at a time (one round trip each) */
+      for (int rb = RR; rb < R[u]; rb += RR) {
+        unsigned c[RR];
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+          c[r] = rb + r < R[u] ? pairs[(size_t)cb[u] + 64 * (rb + r) + lane] : a.e_zero_pair;
+#pragma unroll
+        for (int r = 0; r < RR; ++r)
+          acc = fma(-sl[c[r] & LO], sl[c[r] >> 16], acc);
       }
       if (off[u] >= 0)
         A[off[u]] = acc;

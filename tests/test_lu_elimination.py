@@ -163,8 +163,9 @@ def test_solve_through_plan(chk, m, kmax, max_front):
     th = _theta(P, 0.3, 7)
     b = np.random.default_rng(8).normal(size=m)
     y = np.zeros(m)
-    chk.elim_check_solve(m, len(r), r.ctypes.data_as(ip), c.ctypes.data_as(ip), ix.ctypes.data_as(ip),
-                         th.ctypes.data_as(dp), max_front, b.ctypes.data_as(dp), y.ctypes.data_as(dp))
+    rc = chk.elim_check_solve(m, len(r), r.ctypes.data_as(ip), c.ctypes.data_as(ip), ix.ctypes.data_as(ip),
+                              th.ctypes.data_as(dp), max_front, b.ctypes.data_as(dp), y.ctypes.data_as(dp))
+    assert rc == 0
     A = np.eye(m)
     A[r, c] = -th[ix]
     ref = np.linalg.solve(A, b)
