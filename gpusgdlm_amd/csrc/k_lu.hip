@@ -326,6 +326,107 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   __syncthreads();
 }
 
+/* Rows of the workspace one block step of dense_substitute reads: the
+ * block column's 16-entry segments of the thread's rows (RS rows per
+ * thread, M <= RS NT) and, for lanes 0..15 of wavefront 0, the row of
+ * the diagonal block.  Step st >= M/16: zeros. */
+template <bool FWD, int RS, typename T, int NT>
+__device__ __forceinline__ void substitute_fetch(const T *A, int M, int st, int tid,
+                                                 T (&dst)[RS + 1][16]) {
+  typedef typename Vec16<T>::type VV;
+  constexpr int VW = Vec16<T>::n;
+  const int NBK = M / 16;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int kb = FWD ? st : NBK - 1 - st, b0 = 16 * kb;
+  const bool live = st < NBK;
+#pragma unroll
+  for (int rr = 0; rr <= RS; ++rr) {
+    int i;
+    bool ok;
+    if (rr < RS) {
+      i = FWD ? b0 + 16 + tid + rr * NT : tid + rr * NT;
+      ok = live && (FWD ? i < M : i < b0);
+    } else {
+      i = b0 + lane;
+      ok = live && wave == 0 && lane < 16;
+    }
+    const VV *src = reinterpret_cast<const VV *>(A + (ok ? (size_t)i * M + b0 : 0));
+#pragma unroll
+    for (int c4 = 0; c4 < 16 / VW; ++c4) {
+      VV x = {};
+      if (ok)
+        x = src[c4];
+#pragma unroll
+      for (int e = 0; e < VW; ++e)
+        dst[rr][c4 * VW + e] = x[e];
+    }
+  }
+}
+
+/* One block step st of dense_substitute with its fetched rows buf; then
+ * refills buf with those of step st + DP. */
+template <bool FWD, typename T, int NT, int RS, int DP>
+__device__ __forceinline__ void substitute_step(const T *A, int M, double *xd, int tid, int st,
+                                                T (&buf)[RS + 1][16]) {
+  const int NBK = M / 16;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int b0 = 16 * (FWD ? st : NBK - 1 - st);
+  if (wave == 0 && lane < 16) {
+    /* the diagonal block: 16-step substitution across lanes 0..15 */
+    double x = xd[b0 + lane];
+    if (FWD) {
+#pragma unroll
+      for (int t = 0; t < 15; ++t) {
+        const double xt = __shfl(x, t, 16);
+        if (lane > t)
+          x -= (double)buf[RS][t] * xt;
+      }
+    } else {
+      double dgv = 0.0;
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        dgv = (t == lane) ? (double)buf[RS][t] : dgv;
+#pragma unroll
+      for (int t = 15; t >= 0; --t) {
+        if (lane == t)
+          x = x / dgv;
+        const double xt = __shfl(x, t, 16);
+        if (lane < t)
+          x -= (double)buf[RS][t] * xt;
+      }
+    }
+    xd[b0 + lane] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < RS; ++rr) {
+    const int i = FWD ? b0 + 16 + tid + rr * NT : tid + rr * NT;
+    if (FWD ? i < M : i < b0) {
+      double acc = xd[i];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        acc -= (double)buf[rr][t] * xd[b0 + t];
+      xd[i] = acc;
+    }
+  }
+  substitute_fetch<FWD, RS, T, NT>(A, M, st + DP, tid, buf);
+  __syncthreads();
+}
+
+/* Dense forward (FWD: unit lower L22) or backward (U22) substitution on
+ * xd (LDS, FP64), in block steps of 16 (block kb = st forward,
+ * M/16 - 1 - st backward).  The rows a step reads are fetched at the end
+ * of the previous step, so their latency overlaps its barrier and the
+ * diagonal block's substitution. */
+template <bool FWD, typename T, int NT, int RS>
+__device__ void dense_substitute(const T *A, int M, double *xd, int tid) {
+  const int NBK = M / 16;
+  T buf[RS + 1][16];
+  substitute_fetch<FWD, RS, T, NT>(A, M, 0, tid, buf);
+  for (int st = 0; st < NBK; ++st)
+    substitute_step<FWD, T, NT, RS, 1>(A, M, xd, tid, st, buf);
+}
+
 /* y = (I - Gamma)^{-1} rhs of draw d from the factors the fast kernel
  * leaves: prefix L and U in the LDS slots, dense L22 / U22 of the Schur
  * complement in the workspace A (M x M).  x (n1 + M doubles of LDS) is
@@ -379,55 +480,65 @@ __device__ void prefix_dense_solve(const T *A, int M, char *smem, const double *
       xd[s2] = acc;
     }
     __syncthreads();
-    /* dense: L22 (unit lower) then U22, in blocks of 16 */
-    for (int kb = 0; kb < M / 16; ++kb) {
-      const int b0 = 16 * kb;
-      if (wave == 0 && lane < 16) {
-        double x = xd[b0 + lane];
-        const size_t rowoff = (size_t)(b0 + lane) * M;
-        for (int t = 0; t < 15; ++t) {
-          const double xt = __shfl(x, t, 16);
-          if (lane > t)
-            x -= (double)A[rowoff + b0 + t] * xt;
+    /* dense: L22 (unit lower) then U22 */
+    if constexpr (sizeof(T) == 8) {
+      /* FP64 factors: plain block steps (the prefetch buffers would cost
+       * the FP64 kernel registers in its factorization phases) */
+      for (int kb = 0; kb < M / 16; ++kb) {
+        const int b0 = 16 * kb;
+        if (wave == 0 && lane < 16) {
+          double x = xd[b0 + lane];
+          const size_t rowoff = (size_t)(b0 + lane) * M;
+          for (int t = 0; t < 15; ++t) {
+            const double xt = __shfl(x, t, 16);
+            if (lane > t)
+              x -= (double)A[rowoff + b0 + t] * xt;
+          }
+          xd[b0 + lane] = x;
         }
-        xd[b0 + lane] = x;
-      }
-      __syncthreads();
-      for (int i = b0 + 16 + tid; i < M; i += NT) {
-        const T *Lr = A + (size_t)i * M + b0;
-        double acc = xd[i];
+        __syncthreads();
+        for (int i = b0 + 16 + tid; i < M; i += NT) {
+          const T *Lr = A + (size_t)i * M + b0;
+          double acc = xd[i];
 #pragma unroll
-        for (int t = 0; t < 16; ++t)
-          acc -= (double)Lr[t] * xd[b0 + t];
-        xd[i] = acc;
-      }
-      __syncthreads();
-    }
-    for (int kb = M / 16 - 1; kb >= 0; --kb) {
-      const int b0 = 16 * kb;
-      if (wave == 0 && lane < 16) {
-        const size_t rowoff = (size_t)(b0 + lane) * M;
-        double x = xd[b0 + lane];
-        const double dgv = (double)A[rowoff + b0 + lane];
-        for (int t = 15; t >= 0; --t) {
-          if (lane == t)
-            x = x / dgv;
-          const double xt = __shfl(x, t, 16);
-          if (lane < t)
-            x -= (double)A[rowoff + b0 + t] * xt;
+          for (int t = 0; t < 16; ++t)
+            acc -= (double)Lr[t] * xd[b0 + t];
+          xd[i] = acc;
         }
-        xd[b0 + lane] = x;
+        __syncthreads();
       }
-      __syncthreads();
-      for (int i = tid; i < b0; i += NT) {
-        const T *Ur = A + (size_t)i * M + b0;
-        double acc = xd[i];
+      for (int kb = M / 16 - 1; kb >= 0; --kb) {
+        const int b0 = 16 * kb;
+        if (wave == 0 && lane < 16) {
+          const size_t rowoff = (size_t)(b0 + lane) * M;
+          double x = xd[b0 + lane];
+          const double dgv = (double)A[rowoff + b0 + lane];
+          for (int t = 15; t >= 0; --t) {
+            if (lane == t)
+              x = x / dgv;
+            const double xt = __shfl(x, t, 16);
+            if (lane < t)
+              x -= (double)A[rowoff + b0 + t] * xt;
+          }
+          xd[b0 + lane] = x;
+        }
+        __syncthreads();
+        for (int i = tid; i < b0; i += NT) {
+          const T *Ur = A + (size_t)i * M + b0;
+          double acc = xd[i];
 #pragma unroll
-        for (int t = 0; t < 16; ++t)
-          acc -= (double)Ur[t] * xd[b0 + t];
-        xd[i] = acc;
+          for (int t = 0; t < 16; ++t)
+            acc -= (double)Ur[t] * xd[b0 + t];
+          xd[i] = acc;
+        }
+        __syncthreads();
       }
-      __syncthreads();
+    } else if (M <= NT) {
+      dense_substitute<true, T, NT, 1>(A, M, xd, tid);
+      dense_substitute<false, T, NT, 1>(A, M, xd, tid);
+    } else {
+      dense_substitute<true, T, NT, 2>(A, M, xd, tid);
+      dense_substitute<false, T, NT, 2>(A, M, xd, tid);
     }
     /* backward through the prefix, waves in reverse order */
     for (int w = a.e_waves - 1; w >= 0; --w) {
