@@ -9,12 +9,11 @@
  * batchedComputeNuPlusMu (kernel_functions.cuh:1155-1178) and
  * evolve_lambdas_and_thetas (SGDLM.cu:679-762).
  *
- * Thread mapping: blockIdx.x = series j, consecutive threads = consecutive
+ * Thread mapping: blockIdx.y = series j, consecutive threads = consecutive
  * draws of that series, so a wavefront shares p_j, n_j, s_j and reads the
  * same Cholesky factor (broadcast loads) with no loop divergence except the
  * gamma sampler's rare second attempt.  Per-thread accumulators live in LDS
- * with the thread index as the fastest dimension (odd row stride); k_draw
- * then writes each draw's theta segment with consecutive threads.
+ * with the thread index as the fastest dimension.
  * Random numbers come from counter-based Philox, so every (draw, series)
  * value is independent of launch shape and identical to the host oracle.
  */
@@ -27,17 +26,15 @@ namespace sgdlm {
 namespace {
 
 constexpr int TB = 128;
-constexpr int TS = TB + 1; /* LDS stride of per-thread accumulators (odd:
-                              row and column reads spread over the banks) */
-constexpr int MAX_GRID_Y = 65535;
+constexpr int TS = TB + 1; /* LDS stride of per-thread accumulators */
 
-/* Grids of the per-(draw, series) kernels: blockIdx.x = series j (fastest,
- * so the workgroups in flight write neighbouring series of the same draws
- * and complete whole cache lines of the draw-major theta rows), blockIdx.y =
- * blocks of TB draws, strided when there are more than MAX_GRID_Y. */
+/* Grids of the per-(draw, series) kernels: blockIdx.x = blocks of TB
+ * draws (fastest: the workgroups in flight share a series' Cholesky factor
+ * and parameters), blockIdx.y = series j (m < 46341 by the m^2 < 2^31
+ * bound of sgdlm_init_memory, within the 65535 limit of a grid dimension). */
 dim3 grid_nm(long N, int m) {
   const long nblk = (N + TB - 1) / TB;
-  return dim3((unsigned)m, (unsigned)(nblk < MAX_GRID_Y ? nblk : MAX_GRID_Y));
+  return dim3((unsigned)nblk, (unsigned)m);
 }
 
 /* One workgroup (one wavefront) per series: the right-looking Cholesky of
@@ -92,28 +89,16 @@ __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
                                              uint32_t epoch, double *lam,
                                              double *theta) {
   extern __shared__ double sacc[]; /* [max_p][TS] */
-  const int j = blockIdx.x;
+  const int j = blockIdx.y;
   const int pj = (int)p[j];
-  const long nblk = (N + TB - 1) / TB;
-  for (long ib = blockIdx.y; ib < nblk; ib += gridDim.y) {
-    const long i0 = ib * TB;
-    const int nb = (int)(N - i0 < TB ? N - i0 : TB);
-    const long i = i0 + threadIdx.x;
-    if ((int)threadIdx.x < nb) {
-      const double l = sg_draw_lambda_theta(
-          pj, max_p, m_t + (size_t)j * max_p, U + (size_t)j * max_p * max_p, n_t[j],
-          s_t[j], seed, epoch, (uint32_t)i, (uint32_t)j, sacc + threadIdx.x, TS);
-      lam[(size_t)i * m + j] = l;
-    }
-    __syncthreads();
-    /* the block's segments theta[i][off_j .. off_j + p_j): consecutive
-     * threads write consecutive coefficients */
-    double *base = theta + (size_t)i0 * P + off[j];
-    for (int e = threadIdx.x; e < nb * pj; e += TB) {
-      const int il = e / pj, k = e - il * pj;
-      base[(size_t)il * P + k] = sacc[k * TS + il];
-    }
-    __syncthreads();
+  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
+    const double l = sg_draw_lambda_theta(
+        pj, max_p, m_t + (size_t)j * max_p, U + (size_t)j * max_p * max_p, n_t[j],
+        s_t[j], seed, epoch, (uint32_t)i, (uint32_t)j, sacc + threadIdx.x, TS);
+    lam[(size_t)i * m + j] = l;
+    double *out = theta + (size_t)i * P + off[j];
+    for (int k = 0; k < pj; ++k)
+      out[k] = sacc[k * TS + threadIdx.x];
   }
 }
 
@@ -124,10 +109,10 @@ __global__ __launch_bounds__(TB) void k_forecast_rhs(
     const int *off, long P, const double *x, const double *m_t,
     const double *theta, const double *lam, uint64_t seed, uint32_t epoch,
     double *rhs) {
-  const int j = blockIdx.x;
+  const int j = blockIdx.y;
   const int pj = (int)p[j];
   const unsigned mm = (unsigned)m * (unsigned)m;
-  for (long i = (long)blockIdx.y * TB + threadIdx.x; i < N; i += (long)gridDim.y * TB) {
+  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
     const double *th = theta + (size_t)i * P + off[j];
     double nu = sg_normal_k(seed, epoch, SG_PURPOSE_NU, (uint32_t)i, (uint32_t)j, 0u);
     nu = nu / __builtin_sqrt(lam[(size_t)i * m + j]);
@@ -195,8 +180,8 @@ __global__ __launch_bounds__(TB) void k_evolve(
     const double *beta, const double *G, const double *m_t, uint64_t seed,
     uint32_t epoch, double *lam, double *theta) {
   extern __shared__ double sbuf[]; /* omega [max_p][TB], old theta [max_p][TB] */
-  const int j = blockIdx.x;
-  for (long i = (long)blockIdx.y * TB + threadIdx.x; i < N; i += (long)gridDim.y * TB)
+  const int j = blockIdx.y;
+  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB)
     evolve_one(i, j, m, max_p, p, off, P, U_W, r_t, c_t, beta, G, m_t, seed, epoch, lam, theta,
                sbuf + threadIdx.x, sbuf + (size_t)max_p * TB + threadIdx.x);
 }
@@ -207,9 +192,9 @@ __global__ __launch_bounds__(TB) void k_unpack_theta(long N, int m, int max_p,
                                                      const double *theta,
                                                      const double *m_t,
                                                      double *out) {
-  const int j = blockIdx.x;
+  const int j = blockIdx.y;
   const int pj = (int)p[j];
-  for (long i = (long)blockIdx.y * TB + threadIdx.x; i < N; i += (long)gridDim.y * TB) {
+  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
     const double *th = theta + (size_t)i * P + off[j];
     double *o = out + ((size_t)i * m + j) * max_p;
     for (int k = 0; k < max_p; ++k)
