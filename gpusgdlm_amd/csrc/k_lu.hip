@@ -1437,6 +1437,15 @@ void set_fast_layout(LuArgs &a, const Structure &S, const LuPlan &plan) {
 
 } // namespace
 
+int lu_dense_cap(size_t lds_max) {
+  int cap = 0;
+  for (int M2 = 16; M2 <= 1024; M2 += 16)
+    if (fast_lds_bytes(M2, 64, MODE_NATIVE, 4) <= lds_max / 2 &&
+        fast_lds_bytes(M2, 64, MODE_NATIVE, 8) <= lds_max)
+      cap = M2;
+  return cap;
+}
+
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             int want_threads, int want_nb, LuPlan *plan) {
   const int M = S.M;

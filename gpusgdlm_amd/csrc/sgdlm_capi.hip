@@ -63,6 +63,8 @@ struct sgdlm_ctx {
 
   int prec = SGDLM_PREC_FP64;
   int max_front = 32; /* sparse LU prefix: largest pivot degree (0 = off) */
+  bool max_front_fixed = false; /* set by sgdlm_set_lu_max_front          */
+  int max_front_used = 0;       /* front limit of the current plan        */
   int lu_threads = 0, lu_nb = 0; /* fast LU kernel tuning (0 = automatic) */
   uint64_t seed = 1234ull;
   uint32_t epoch = 0;
@@ -287,12 +289,35 @@ template <typename V> int upload(sgdlm_ctx *c, const char *name, const V &v) {
 int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
   ElimPlan P;
   const size_t budget = c->lds_max > 1024 ? c->lds_max - 1024 : 0;
-  for (int mf = c->max_front;; mf /= 2) {
-    build_elim_plan((int)c->m, ents, mf, (int)std::min<size_t>(65536, budget / 8), P);
-    const size_t need = (size_t)P.ns * 8 + 16 + P.tab.size() * 4;
-    if (need <= budget || mf == 0)
+  const int max_slots = (int)std::min<size_t>(65536, budget / 8);
+  auto fits = [&](const ElimPlan &Q) {
+    return (size_t)Q.ns * 8 + 16 + Q.tab.size() * 4 <= budget;
+  };
+  int mf = c->max_front;
+  for (;; mf /= 2) {
+    build_elim_plan((int)c->m, ents, mf, max_slots, P);
+    if (fits(P) || mf == 0)
       break;
   }
+  /* Unless the front limit was set explicitly: when the dense block just
+   * misses the size up to which the 64-wide fast LU fits its LDS budget, a
+   * somewhat larger front (a few more prefix eliminations) usually brings
+   * it under; that costs far less than the narrower panels would. */
+  const int cap = lu_dense_cap(c->lds_max);
+  if (!c->max_front_fixed && mf == c->max_front && P.M2 > cap && P.M2 <= cap + 64) {
+    for (int mf2 = mf + 4; mf2 <= 64; mf2 += 4) {
+      ElimPlan Q;
+      build_elim_plan((int)c->m, ents, mf2, max_slots, Q);
+      if (!fits(Q))
+        break;
+      if (Q.M2 <= cap) {
+        P = std::move(Q);
+        mf = mf2;
+        break;
+      }
+    }
+  }
+  c->max_front_used = mf;
   int rc;
   if ((rc = upload(c, "e_slot_init", P.slot_init)) || (rc = upload(c, "e_tab", P.tab)) ||
       (rc = upload(c, "e_sg_off", P.sg_off)) || (rc = upload(c, "e_sg_idx", P.sg_idx)) ||
@@ -597,7 +622,10 @@ int sgdlm_manage_evo_memory(sgdlm_ctx *c, int use) {
       int rc = dalloc(c, c->mem, "G", c->m * c->max_p * c->max_p * 8);
       if (rc)
         return rc;
-      HIPCHK(c, hipMemset(c->mem["G"].ptr, 0, c->mem["G"].bytes));
+      /* on the context's stream (non-blocking: a null-stream memset could
+       * land after a later G upload on it) */
+      HIPCHK(c, hipMemsetAsync(c->mem["G"].ptr, 0, c->mem["G"].bytes, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
     }
   } else {
     auto it = c->mem.find("G");
@@ -1122,7 +1150,9 @@ int sgdlm_get_lu_fallback_count(const sgdlm_ctx *cc, long *count) {
   if (!c->lu_flags_valid || !c->sim_memory_initialized)
     return fail(c, SGDLM_ERR_STATE, "no VB or forecast LU has run");
   int n = 0;
-  HIPCHK(c, hipMemcpy(&n, SIM<int>(c, "lu_flags"), sizeof(int), hipMemcpyDeviceToHost));
+  int rc;
+  if ((rc = d2h(c, &n, SIM<int>(c, "lu_flags"), sizeof(int))) || (rc = sync_and_collect(c)))
+    return rc;
   *count = n;
   return SGDLM_OK;
 }
@@ -1165,6 +1195,7 @@ int sgdlm_set_lu_max_front(sgdlm_ctx *c, int max_front) {
   if (max_front < 0)
     return fail(c, SGDLM_ERR_ARG, "max_front must be >= 0");
   c->max_front = max_front;
+  c->max_front_fixed = true;
   c->vb_valid = false;
   if (c->parents_set)
     return build_structure(c);
@@ -1176,7 +1207,7 @@ int sgdlm_get_lu_elimination(const sgdlm_ctx *c, int *max_front, int *n_prefix, 
   if (!c)
     return SGDLM_ERR_ARG;
   if (max_front)
-    *max_front = c->max_front;
+    *max_front = c->max_front_used;
   if (!c->parents_set)
     return SGDLM_ERR_STATE;
   if (n_prefix)

@@ -90,6 +90,9 @@ struct LuPlan {
 };
 /* want_threads / want_nb: 0 = automatic, else the fast kernel's workgroup
  * size (256 or 512) and panel width; an infeasible request fails. */
+/* Largest dense Schur-complement size (multiple of 16) for which the
+ * 64-wide fast LU fits: FP32 in half a CU's LDS, FP64 in a whole CU's. */
+int lu_dense_cap(size_t lds_max);
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             int want_threads, int want_nb, LuPlan *plan);
 /* flags: N + 1 ints of scratch (count + list of draws handed to the

@@ -237,6 +237,33 @@ def test_logdet_sparse_prefix_m400(hiplib, prec, tol):
     assert fb <= N // 8
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 5e-3)])
+def test_front_raised_for_wide_panels(hiplib, prec, tol):
+    """With the default front limit, an m = 400 structure whose dense block
+    would be 304 at front 32 (above the 288 up to which the 64-wide fast LU
+    fits in LDS) is planned with a slightly larger front: dense block <= 288,
+    64-wide panels, and the oracle's log-determinants."""
+    mod = random_model(400, 20, T=45, seed=1)
+    N = 128
+    print(f"front raise m=400 seed 1 {prec} N={N}", flush=True)
+    g, o = make_pair(mod, prec)
+    filter_steps(g, mod, 1, 40)
+    g.computePosterior(mod.y[40], mod.F(40))
+    g.computeVBPosterior(N, N)
+    lw = g.get_vb_result()["log_weights"]
+    el, cfg, fb = g.lu_elimination(), g.lu_config(), g.lu_fallback_count()
+    g.close()
+    print(f"  {el} {cfg}")
+    assert el["max_front"] > 32 and el["n_dense"] <= 288 and cfg["nb"] == 64
+    filter_steps(o, mod, 1, 40)
+    o.computePosterior(mod.y[40], mod.F(40))
+    prm = o.getParameters()
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
+    assert np.max(np.abs(lw - ld)) <= tol
+    assert fb <= N // 8
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 5e-3)])
 def test_logdet_sparse_prefix_m1000(hiplib, prec, tol):
     """m = 1000, k ~ U{0..30}: dense Schur complement of ~750 series (two
