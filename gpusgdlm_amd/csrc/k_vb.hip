@@ -132,162 +132,75 @@ __global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
   }
 }
 
-/* pass 1: per (chunk, series) sums of w*lambda*theta_k, w*lambda,
- * w*log(lambda).  threads = 4 draw groups x 64 coefficient lanes. */
-/* pass 1, draw-major: workgroup (chunk c, part s) streams the packed draw
- * rows of its draw range (coalesced) and accumulates, per packed
- * coefficient q (series j(q)), sum w lambda_j theta_q in registers, QU
- * coefficients per thread per sweep; threads < m also sum w lambda_j and
- * w log lambda_j.  Partials go to part1s[(c * MOM1_SPLIT + s) * (P + 2m)]. */
-constexpr int MOM1_SPLIT = 64;
-constexpr int MOM1_QU = 8;
-
-__global__ __launch_bounds__(256) void k_mom1(long N, int m, int chunks, const int *qser,
-                                              long P, const double *w, const double *lam,
-                                              const double *theta, double *part1s) {
-  const int c = blockIdx.x / MOM1_SPLIT, sp = blockIdx.x % MOM1_SPLIT;
-  const long c0 = N * c / chunks, c1 = N * (c + 1) / chunks;
-  const long i0 = c0 + (c1 - c0) * sp / MOM1_SPLIT, i1 = c0 + (c1 - c0) * (sp + 1) / MOM1_SPLIT;
-  const int tid = threadIdx.x;
-  double *out = part1s + (size_t)blockIdx.x * (size_t)(P + 2 * m);
-  for (int j = tid; j < m; j += 256) {
-    double al = 0.0, ll = 0.0;
-#pragma unroll 4
-    for (long i = i0; i < i1; ++i) {
-      const double wi = w[i];
-      const double li = lam[(size_t)i * m + j];
-      al += wi * li;
-      ll += wi * log(li);
-    }
-    out[P + j] = al;
-    out[P + m + j] = ll;
-  }
-  for (long qb = 0; qb < P; qb += 256 * MOM1_QU) {
-    long q[MOM1_QU];
-    int js[MOM1_QU];
-    double acc[MOM1_QU];
-#pragma unroll
-    for (int u = 0; u < MOM1_QU; ++u) {
-      q[u] = qb + tid + 256 * u;
-      js[u] = q[u] < P ? qser[q[u]] : 0;
-      acc[u] = 0.0;
-    }
-#pragma unroll 4
-    for (long i = i0; i < i1; ++i) {
-      const double wi = w[i];
-      const double *row = theta + (size_t)i * P;
-      const double *lrow = lam + (size_t)i * m;
-#pragma unroll
-      for (int u = 0; u < MOM1_QU; ++u) {
-        if (q[u] < P) {
-          const double wl = wi * lrow[js[u]];
-          acc[u] += wl * row[q[u]];
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < MOM1_QU; ++u)
-      if (q[u] < P)
-        out[q[u]] = acc[u];
-  }
-}
-
-/* pass-1 partials of chunk c in the per-series layout
- * part1[(c m + j)(max_p + 2) + k]: k < p_j coefficient sums, max_p: sum w
- * lambda, max_p + 1: sum w log lambda; parts added in a fixed order. */
-__global__ __launch_bounds__(256) void k_mom1_combine(int m, int max_p, const unsigned *p,
-                                                      const int *off, long P,
-                                                      const double *part1s, double *part1) {
-  const int c = blockIdx.y;
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= m * (max_p + 2))
-    return;
-  const int j = e / (max_p + 2), k = e - j * (max_p + 2);
-  long src = -1;
-  if (k < (int)p[j])
-    src = off[j] + k;
-  else if (k == max_p)
-    src = P + j;
-  else if (k == max_p + 1)
-    src = P + m + j;
-  double acc = 0.0;
-  if (src >= 0) {
-    const double *ps = part1s + (size_t)c * MOM1_SPLIT * (size_t)(P + 2 * m) + src;
-    for (int s2 = 0; s2 < MOM1_SPLIT; ++s2)
-      acc += ps[(size_t)s2 * (size_t)(P + 2 * m)];
-  }
-  part1[(size_t)c * m * (max_p + 2) + e] = acc;
-}
-
-/* combine pass-1 partials of series j in chunk order */
-__device__ void series_means(int j, int m, int max_p, int chunks, int pj,
-                             const double *part1, int k, double *lbar,
-                             double *llbar, double *mk) {
-  double sl = 0.0, sll = 0.0, st = 0.0;
-  for (int c = 0; c < chunks; ++c) {
-    const double *pp = part1 + ((size_t)c * m + j) * (max_p + 2);
-    sl += pp[max_p];
-    sll += pp[max_p + 1];
-    if (k < pj)
-      st += pp[k];
-  }
-  *lbar = sl;
-  *llbar = sll;
-  *mk = (k < pj) ? st / sl : 0.0;
-}
-
-/* pass 2: V partials on FP64 MFMA.  NBLK = number of 16-wide coefficient
- * blocks (p_j <= 16 NBLK). */
+/* One pass over the draws of chunk c for series j, on FP64 MFMA
+ * (v_mfma_f64_16x16x4f64, p x N . N x p).  With x_k = sqrt(w lambda)
+ * (theta_k - c_k) for k < p_j and x_{p_j} = sqrt(w lambda), X = sum x x^T
+ * holds sum w lambda (theta - c)(theta - c)^T, sum w lambda (theta - c)
+ * (column p_j) and sum w lambda (entry (p_j, p_j)); the shift c is the
+ * naive posterior mean the draws are centred on, so forming
+ * sum w lambda (theta - mbar)(theta - mbar)^T from X later cancels little.
+ * Also sum w log lambda (part1).  NBLK = number of 16-wide coefficient
+ * blocks, p_j + 1 <= 16 NBLK.  X goes to part2[(c m + j) (max_p+1)^2]. */
 template <int NBLK>
-__global__ __launch_bounds__(256) void k_mom2(long N, int m, int max_p,
-                                              int chunks, const unsigned *p,
-                                              const int *off, long P,
-                                              const double *w,
-                                              const double *lam,
-                                              const double *theta,
-                                              const double *part1,
-                                              double *part2) {
+__global__ __launch_bounds__(256) void k_mom(long N, int m, int max_p, int chunks,
+                                             const unsigned *p, const int *off, long P,
+                                             const double *w, const double *lam,
+                                             const double *theta, const double *m_t,
+                                             double *part1, double *part2) {
   constexpr int NT = NBLK * (NBLK + 1) / 2;
-  __shared__ double mb[16 * NBLK];
   __shared__ double tiles[NT][256];
+  __shared__ double red[4];
   const int j = blockIdx.y, c = blockIdx.x;
   const long i0 = N * c / chunks, i1 = N * (c + 1) / chunks;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, g = lane >> 4;
-  const int pj = (int)p[j];
-  if (tid < 16 * NBLK) {
-    double lb, llb, mk;
-    series_means(j, m, max_p, chunks, pj, part1, tid, &lb, &llb, &mk);
-    mb[tid] = mk;
+  const int pj = (int)p[j], mq = max_p + 1;
+  double cb[NBLK];
+#pragma unroll
+  for (int b = 0; b < NBLK; ++b) {
+    const int k = 16 * b + l16;
+    cb[b] = (k < pj) ? m_t[(size_t)j * max_p + k] : 0.0;
   }
-  __syncthreads();
   f64x4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t)
     acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double wl = 0.0;
   const double *th = theta + off[j];
-  const long nsteps = (i1 - i0 + 3) / 4;
-#pragma unroll 4
-  for (long s = wave; s < nsteps; s += 4) {
-    const long i = i0 + 4 * s + g;
-    double x[NBLK];
-    double sc = 0.0;
-    if (i < i1)
-      sc = sqrt(w[i] * lam[(size_t)i * m + j]);
-#pragma unroll
-    for (int b = 0; b < NBLK; ++b) {
-      const int k = 16 * b + l16;
-      x[b] = (i < i1 && k < pj) ? sc * (th[(size_t)i * P + k] - mb[k]) : 0.0;
+  /* each wavefront takes 64 draws at a time: lane l computes sqrt(w lambda)
+   * and w log lambda of draw base + l once, then 16 MFMA steps of 4 draws
+   * each read it back by lane shuffle */
+  for (long base = i0 + 64 * wave; base < i1; base += 256) {
+    const long id = base + lane;
+    double scl = 0.0;
+    if (id < i1) {
+      const double wi = w[id], li = lam[(size_t)id * m + j];
+      scl = sqrt(wi * li);
+      wl += wi * log(li);
     }
-    int t = 0;
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+      const long i = base + 4 * q + g;
+      const double sc = __shfl(scl, 4 * q + g);
+      double x[NBLK];
 #pragma unroll
-    for (int A = 0; A < NBLK; ++A)
-#pragma unroll
-      for (int B = A; B < NBLK; ++B) {
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[A], x[B], acc[t], 0, 0, 0);
-        ++t;
+      for (int b = 0; b < NBLK; ++b) {
+        const int k = 16 * b + l16;
+        x[b] = (i < i1 && k < pj) ? sc * (th[(size_t)i * P + k] - cb[b]) : (k == pj ? sc : 0.0);
       }
+      int t = 0;
+#pragma unroll
+      for (int A = 0; A < NBLK; ++A)
+#pragma unroll
+        for (int B = A; B < NBLK; ++B) {
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[A], x[B], acc[t], 0, 0, 0);
+          ++t;
+        }
+    }
   }
+  wl = wave_sum(wl);
+  if (lane == 0)
+    red[wave] = wl;
   /* sum the 4 waves' tiles in a fixed order */
   for (int wv = 0; wv < 4; ++wv) {
     if (wave == wv) {
@@ -303,16 +216,16 @@ __global__ __launch_bounds__(256) void k_mom2(long N, int m, int max_p,
     }
     __syncthreads();
   }
-  double *out = part2 + ((size_t)c * m + j) * max_p * max_p;
+  if (tid == 0)
+    part1[(size_t)c * m + j] = ((red[0] + red[1]) + red[2]) + red[3];
+  double *out = part2 + ((size_t)c * m + j) * mq * mq;
   int t = 0;
   for (int A = 0; A < NBLK; ++A)
     for (int B = A; B < NBLK; ++B) {
-      for (int e = tid; e < 256; e += 256) {
-        const int r = 16 * A + e / 16, col = 16 * B + (e & 15);
-        if (r < pj && col < pj) {
-          out[r + (size_t)col * max_p] = tiles[t][e];
-          out[col + (size_t)r * max_p] = tiles[t][e];
-        }
+      const int r = 16 * A + tid / 16, col = 16 * B + (tid & 15);
+      if (r <= pj && col <= pj) {
+        out[r + (size_t)col * mq] = tiles[t][tid];
+        out[col + (size_t)r * mq] = tiles[t][tid];
       }
       ++t;
     }
@@ -335,34 +248,42 @@ __device__ double psi1_series(double x) {
 
 /* final per-series moment-match; one wavefront per series */
 __global__ __launch_bounds__(64) void k_vb_final(
-    int m, int max_p, int chunks, const unsigned *p, const double *n_t,
+    int m, int max_p, int chunks, const unsigned *p, const double *m_t, const double *n_t,
     const double *s_t, const double *part1, const double *part2,
     double *mean_lambda, double *mean_log_lambda, double *mean_m,
     double *mean_C, double *mean_n, double *mean_s, double *mean_Q,
     int *newton_iter) {
   extern __shared__ double sh[];
   const int j = blockIdx.x, t = threadIdx.x;
-  const int pj = (int)p[j], mp = max_p, mp2 = max_p * max_p;
+  const int pj = (int)p[j], mp = max_p, mp2 = max_p * max_p, mq = max_p + 1;
   double *V = sh;                     /* pj x pj (ld pj) */
   double *Aug = V + (size_t)pj * pj;  /* pj x 2pj        */
+  double *D = Aug + (size_t)2 * pj * pj; /* pj: sum w lambda (theta - c) */
   __shared__ double sc[4];
   __shared__ int spiv;
-  double lb = 0.0, llb = 0.0, mk = 0.0;
-  for (int k = t; k < mp; k += 64) {
-    series_means(j, m, max_p, chunks, pj, part1, k, &lb, &llb, &mk);
-    mean_m[(size_t)j * mp + k] = mk;
-  }
-  if (t == 0) {
-    series_means(j, m, max_p, chunks, pj, part1, 0, &lb, &llb, &mk);
-    sc[0] = lb;
-    sc[1] = llb;
-  }
-  for (int e = t; e < pj * pj; e += 64) {
-    const int r = e % pj, c = e / pj;
+  /* the chunks' partial sums, added in chunk order */
+  auto xsum = [&](int r, int col) {
     double v = 0.0;
     for (int ch = 0; ch < chunks; ++ch)
-      v += part2[((size_t)ch * m + j) * mp2 + r + (size_t)c * mp];
-    V[e] = v;
+      v += part2[((size_t)ch * m + j) * mq * mq + r + (size_t)col * mq];
+    return v;
+  };
+  if (t == 0) {
+    double l = 0.0;
+    for (int ch = 0; ch < chunks; ++ch)
+      l += part1[(size_t)ch * m + j];
+    sc[0] = xsum(pj, pj);
+    sc[1] = l;
+  }
+  for (int k = t; k < pj; k += 64)
+    D[k] = xsum(k, pj);
+  __syncthreads();
+  const double lb = sc[0];
+  for (int k = t; k < mp; k += 64)
+    mean_m[(size_t)j * mp + k] = (k < pj) ? m_t[(size_t)j * mp + k] + D[k] / lb : 0.0;
+  for (int e = t; e < pj * pj; e += 64) {
+    const int r = e % pj, c = e / pj;
+    V[e] = xsum(r, c) - D[r] * D[c] / lb;
   }
   __syncthreads();
   for (int e = t; e < pj * 2 * pj; e += 64) {
@@ -449,10 +370,6 @@ __global__ __launch_bounds__(64) void k_vb_final(
 
 } // namespace
 
-size_t vb_part1s_doubles(const Structure &S, long N) {
-  return (size_t)vb_chunks(N) * MOM1_SPLIT * (size_t)(S.P + 2 * S.m);
-}
-
 int vb_chunks(long N) {
   long c = N / 2048;
   if (c < 1)
@@ -468,41 +385,38 @@ void launch_weights(long N, const double *logdet, double *w, double *stats,
 }
 
 void launch_moments(const Structure &S, long N, const double *lam,
-                    const double *theta, const double *n_t, const double *s_t,
-                    VbBuffers &B, hipStream_t st) {
+                    const double *theta, const double *m_t, const double *n_t,
+                    const double *s_t, VbBuffers &B, hipStream_t st) {
   const int chunks = B.chunks;
-  hipLaunchKernelGGL(k_mom1, dim3(chunks * MOM1_SPLIT), dim3(256), 0, st, N, S.m, chunks,
-                     S.qser, S.P, B.w, lam, theta, B.part1s);
-  hipLaunchKernelGGL(k_mom1_combine, dim3((S.m * (S.max_p + 2) + 255) / 256, chunks), dim3(256),
-                     0, st, S.m, S.max_p, S.p, S.off, S.P, B.part1s, B.part1);
-  const int nblk = (S.max_p + 15) / 16;
+  const int nblk = (S.max_p + 1 + 15) / 16;
+  const dim3 grid(chunks, S.m);
   switch (nblk) {
   case 1:
-    hipLaunchKernelGGL(k_mom2<1>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
-                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
-                       B.part1, B.part2);
+    hipLaunchKernelGGL(k_mom<1>, grid, dim3(256), 0, st, N, S.m, S.max_p, chunks, S.p, S.off,
+                       S.P, B.w, lam, theta, m_t, B.part1, B.part2);
     break;
   case 2:
-    hipLaunchKernelGGL(k_mom2<2>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
-                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
-                       B.part1, B.part2);
+    hipLaunchKernelGGL(k_mom<2>, grid, dim3(256), 0, st, N, S.m, S.max_p, chunks, S.p, S.off,
+                       S.P, B.w, lam, theta, m_t, B.part1, B.part2);
     break;
   case 3:
-    hipLaunchKernelGGL(k_mom2<3>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
-                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
-                       B.part1, B.part2);
+    hipLaunchKernelGGL(k_mom<3>, grid, dim3(256), 0, st, N, S.m, S.max_p, chunks, S.p, S.off,
+                       S.P, B.w, lam, theta, m_t, B.part1, B.part2);
+    break;
+  case 4:
+    hipLaunchKernelGGL(k_mom<4>, grid, dim3(256), 0, st, N, S.m, S.max_p, chunks, S.p, S.off,
+                       S.P, B.w, lam, theta, m_t, B.part1, B.part2);
     break;
   default:
-    hipLaunchKernelGGL(k_mom2<4>, dim3(chunks, S.m), dim3(256), 0, st, N, S.m,
-                       S.max_p, chunks, S.p, S.off, S.P, B.w, lam, theta,
-                       B.part1, B.part2);
+    hipLaunchKernelGGL(k_mom<5>, grid, dim3(256), 0, st, N, S.m, S.max_p, chunks, S.p, S.off,
+                       S.P, B.w, lam, theta, m_t, B.part1, B.part2);
     break;
   }
-  const size_t lds = 3 * (size_t)S.max_p * S.max_p * sizeof(double);
+  const size_t lds = (3 * (size_t)S.max_p * S.max_p + S.max_p) * sizeof(double);
   (void)hipFuncSetAttribute((const void *)k_vb_final,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k_vb_final, dim3(S.m), dim3(64), lds, st, S.m, S.max_p,
-                     chunks, S.p, n_t, s_t, B.part1, B.part2, B.mean_lambda,
+                     chunks, S.p, m_t, n_t, s_t, B.part1, B.part2, B.mean_lambda,
                      B.mean_log_lambda, B.mean_m, B.mean_C, B.mean_n,
                      B.mean_s, B.mean_Q, B.newton_iter);
 }

@@ -230,9 +230,8 @@ int ensure_sim_buffers(sgdlm_ctx *c) {
       (rc = dalloc(c, c->sim, "lu_flags", (N + 1) * 4)))
     return rc;
   const int chunks = vb_chunks((long)N);
-  if ((rc = dalloc(c, c->sim, "part1", (size_t)chunks * m * (mp + 2) * 8)) ||
-      (rc = dalloc(c, c->sim, "part1s", vb_part1s_doubles(c->S, (long)N) * 8)) ||
-      (rc = dalloc(c, c->sim, "part2", (size_t)chunks * m * mp * mp * 8)) ||
+  if ((rc = dalloc(c, c->sim, "part1", (size_t)chunks * m * 8)) ||
+      (rc = dalloc(c, c->sim, "part2", (size_t)chunks * m * (mp + 1) * (mp + 1) * 8)) ||
       (rc = dalloc(c, c->sim, "mean_lambda", m * 8)) ||
       (rc = dalloc(c, c->sim, "mean_log_lambda", m * 8)) ||
       (rc = dalloc(c, c->sim, "mean_m", m * mp * 8)) ||
@@ -250,7 +249,6 @@ VbBuffers vb_buffers(sgdlm_ctx *c) {
   B.w = SIM<double>(c, "w");
   B.stats = SIM<double>(c, "stats");
   B.part1 = SIM<double>(c, "part1");
-  B.part1s = SIM<double>(c, "part1s");
   B.part2 = SIM<double>(c, "part2");
   B.mean_lambda = SIM<double>(c, "mean_lambda");
   B.mean_log_lambda = SIM<double>(c, "mean_log_lambda");
@@ -392,12 +390,6 @@ int build_structure(sgdlm_ctx *c) {
   for (int r = 0; r < m; ++r)
     rp[r + 1] += rp[r];
   int rc;
-  std::vector<int> qser(std::max(off[m], 1), 0);
-  for (int j = 0; j < m; ++j)
-    for (int k = off[j]; k < off[j + 1]; ++k)
-      qser[k] = j;
-  if ((rc = upload(c, "qser", qser)))
-    return rc;
   if ((rc = dalloc(c, c->mem, "off", (m + 1) * 4)) ||
       (rc = dalloc(c, c->mem, "g_row", std::max(ng, 1) * 4)) ||
       (rc = dalloc(c, c->mem, "g_col", std::max(ng, 1) * 4)) ||
@@ -420,7 +412,6 @@ int build_structure(sgdlm_ctx *c) {
   S.p = M_<unsigned>(c, "p");
   S.sp = M_<unsigned>(c, "sp");
   S.off = M_<int>(c, "off");
-  S.qser = M_<int>(c, "qser");
   S.g_row = M_<int>(c, "g_row");
   S.g_col = M_<int>(c, "g_col");
   S.g_idx = M_<int>(c, "g_idx");
@@ -472,7 +463,7 @@ int run_vb(sgdlm_ctx *c) {
   {
     StageTimer t(c, "moments");
     launch_moments(c->S, N, SIM<double>(c, "lam"), SIM<double>(c, "theta"),
-                   M_<double>(c, "n"), M_<double>(c, "s"), B, c->stream);
+                   M_<double>(c, "m"), M_<double>(c, "n"), M_<double>(c, "s"), B, c->stream);
   }
   CHECK_LAUNCH(c);
   c->vb_valid = true;

@@ -23,7 +23,6 @@ struct Structure {
   const unsigned *p = nullptr;   /* device, m                                */
   const unsigned *sp = nullptr;  /* device, m * max_p                        */
   const int *off = nullptr;      /* device, m + 1 prefix sums of p           */
-  const int *qser = nullptr;     /* device, P: series of each packed index   */
   /* Gamma entries sorted by (row, col): A[row][col] = -theta_packed[idx]   */
   const int *g_row = nullptr, *g_col = nullptr, *g_idx = nullptr;
   const int *g_rowptr = nullptr; /* device, m + 1 (CSR over rows)           */
@@ -110,9 +109,9 @@ struct VbBuffers {
   double *w = nullptr;        /* N normalised weights                      */
   double *stats = nullptr;    /* [0] max logdet, [1] sum exp(l - max),
                                  [2] sum w^2, [3] number of zero weights     */
-  double *part1 = nullptr;    /* chunks * (m * (2 + max_p)) partial sums    */
-  double *part1s = nullptr;   /* finer draw-major pass-1 partials            */
-  double *part2 = nullptr;    /* chunks * (m * max_p^2) partial sums        */
+  double *part1 = nullptr;    /* chunks * m: sum w log lambda               */
+  double *part2 = nullptr;    /* chunks * m * (max_p + 1)^2: shifted second
+                                 moments with the first ones (k_mom)        */
   double *mean_lambda = nullptr, *mean_log_lambda = nullptr; /* m each      */
   double *mean_m = nullptr;   /* m * max_p                                   */
   double *mean_C = nullptr;   /* m * max_p^2                                 */
@@ -121,12 +120,11 @@ struct VbBuffers {
   int chunks = 0;
 };
 int vb_chunks(long N);
-size_t vb_part1s_doubles(const Structure &S, long N);
 void launch_weights(long N, const double *logdet, double *w, double *stats,
                     hipStream_t st);
 void launch_moments(const Structure &S, long N, const double *lam,
-                    const double *theta, const double *n_t, const double *s_t,
-                    VbBuffers &B, hipStream_t st);
+                    const double *theta, const double *m_t, const double *n_t,
+                    const double *s_t, VbBuffers &B, hipStream_t st);
 
 } // namespace sgdlm
 
