@@ -180,10 +180,14 @@ int sgdlm_set_lu_tuning(sgdlm_ctx *ctx, int threads, int nb);
  * (symmetrised) parental graph stays <= max_front are eliminated by a
  * minimum-degree schedule before the dense LU of the remaining series
  * (default 32; 0 = dense LU of the whole I - Gamma).  The result is the
- * same determinant; only the rounding differs. */
+ * same determinant; only the rounding differs.  Unless set here, the limit
+ * is raised in steps of 4 (up to 64) when that brings the dense block
+ * under the size up to which the 64-wide fast LU fits its LDS (288 on
+ * gfx950); a value set here is used as given. */
 int sgdlm_set_lu_max_front(sgdlm_ctx *ctx, int max_front);
-/* The plan for the current parental sets: series eliminated in the prefix,
- * series in the dense LU, prefix waves and prefix multiply-adds per draw. */
+/* The plan for the current parental sets: the front limit it used, series
+ * eliminated in the prefix, series in the dense LU, prefix waves and prefix
+ * multiply-adds per draw. */
 int sgdlm_get_lu_elimination(const sgdlm_ctx *ctx, int *max_front,
                              int *n_prefix, int *n_dense, int *n_waves,
                              long *prefix_fma);
