@@ -8,12 +8,14 @@
  * (SGDLM.cu:556-620), which the reference runs on cuBLAS with the matrices
  * round-tripping HBM.
  *
- * Both kernels are persistent (one workgroup per CU, looping over draws);
- * the matrix of a draw lives in a per-workgroup global workspace (row-major,
- * M x M, M = m rounded up to 16 with identity padding) that stays hot in L2 /
- * Infinity Cache, and is assembled there from the packed coefficient draws.
+ * Both kernels are persistent (one or two workgroups per CU, looping over
+ * draws); the matrix of a draw lives in a per-workgroup global workspace
+ * (row-major, M x M, M = m rounded up to 16 with identity padding) that
+ * stays in L2 / Infinity Cache, and is assembled there from the packed
+ * coefficient draws.
  *
- * k_lu_fast (VB log-determinants; 8 wavefronts):
+ * k_lu_fast (VB log-determinants and forecast solves; two 256-thread
+ * workgroups per CU, or one of 512 threads when its LDS needs a whole CU):
  *   Sparse prefix (tables from lu_symbolic.cpp): the low-degree series of
  *   the parental graph are eliminated first, wave by wave, on LDS slots
  *   holding only the entries their elimination touches; the Schur complement
@@ -26,19 +28,22 @@
  *   tau = 1 and then performs no interchange exactly when all |l_ij| <= 1).
  *   For draws of I - Gamma the multipliers stay near or below 1, so this
  *   kernel factors without interchanges (no pivot searches, one barrier per
- *   column: the panel rows live in registers, the pivot row is broadcast
- *   through LDS), tracks max |l_ij|, and hands any draw with
+ *   column: the panel rows live in registers in sub-panels of 32 columns,
+ *   the pivot row is broadcast through LDS; FP32 multipliers through the
+ *   hardware reciprocal), tracks max |l_ij|, and hands any draw with
  *   max |l| > 1/tau = 2 or a zero pivot to k_lu_pivot through a device-side
  *   list.
  *     - U12 = L11^{-1} A12 per 16-column tile on MFMA from the inverses of
  *       the 16 x 16 diagonal blocks of L11 (written back in place of A12);
  *     - A22 -= L21 U12 on MFMA: f64 16x16x4 (FP64), f32 16x16x4 (FP32), or
  *       bf16 16x16x32 hi*hi + hi*lo + lo*hi with FP32 accumulation (BF16x3);
- *       work items (column tile, row tile) are split into 8 contiguous
- *       ranges and their loads are issued one batch ahead.
+ *       work items (column-tile pair, row tile) are split into one
+ *       contiguous range per wavefront and their loads are issued one batch
+ *       ahead.
  *   The contraction index of every MFMA is permuted so that an accumulator
  *   tile can be reused directly as a B operand (rho() below).
- * k_lu_pivot (fallback and forecast solves; 4 wavefronts):
+ * k_lu_pivot (fallback; 256 threads, two workgroups per CU when the panel
+ *   fits half a CU's LDS):
  *   the same blocked algorithm with partial pivoting in the panel (largest
  *   |a|, lowest row on ties, as LAPACK getf2); interchanges permute a
  *   logical->physical row map in LDS instead of moving rows.  Forecast mode
