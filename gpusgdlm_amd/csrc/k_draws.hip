@@ -102,6 +102,57 @@ __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
   }
 }
 
+/* k_draw with the p_j accumulators in registers (p_j <= PMAX): the same
+ * operations in the same order as sg_draw_lambda_theta, without the LDS
+ * that bounds k_draw's occupancy.  p_j is uniform over a workgroup, so the
+ * unrolled loops' bounds checks are uniform branches. */
+template <int PMAX>
+__global__ __launch_bounds__(TB) void k_draw_reg(long N, int m, int max_p,
+                                                 const unsigned *p, const int *off,
+                                                 long P, const double *m_t,
+                                                 const double *U, const double *n_t,
+                                                 const double *s_t, uint64_t seed,
+                                                 uint32_t epoch, double *lam,
+                                                 double *theta) {
+  const int j = blockIdx.y;
+  const int pj = (int)p[j];
+  const double *Uj = U + (size_t)j * max_p * max_p, *mj = m_t + (size_t)j * max_p;
+  const double nj = n_t[j], sj = s_t[j];
+  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
+    const double l = sg_gamma(0.5 * nj, 0.5 * nj * sj, seed, epoch, SG_PURPOSE_LAMBDA,
+                              (uint32_t)i, (uint32_t)j);
+    const double scale = __builtin_sqrt(sj * l);
+    double acc[PMAX];
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k)
+      acc[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < PMAX; q += 2) {
+      if (q < pj) {
+        double z0, z1;
+        sg_normal_pair(sg_rng_block(seed, epoch, SG_PURPOSE_THETA, (uint32_t)i, (uint32_t)j,
+                                    (uint32_t)(q >> 1)),
+                       &z0, &z1);
+#pragma unroll
+        for (int k = q; k < PMAX; ++k) {
+          if (k < pj) {
+            double t = acc[k] + Uj[k * max_p + q] * z0;
+            if (q + 1 <= k)
+              t = t + Uj[k * max_p + q + 1] * z1;
+            acc[k] = t;
+          }
+        }
+      }
+    }
+    lam[(size_t)i * m + j] = l;
+    double *out = theta + (size_t)i * P + off[j];
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k)
+      if (k < pj)
+        out[k] = mj[k] + acc[k] / scale;
+  }
+}
+
 /* rhs_ij = nu_ij / sqrt(lambda_ij) + sum_{k: sp_jk external} x_jk theta_ijk
  * (kernel_functions.cuh:1155-1178; theta_k = m_k beyond p_j). */
 __global__ __launch_bounds__(TB) void k_forecast_rhs(
@@ -214,6 +265,13 @@ void launch_draw(const Structure &S, long N, const double *m_t,
                  const double *U, const double *n_t, const double *s_t,
                  uint64_t seed, uint32_t epoch, double *lam, double *theta,
                  hipStream_t st) {
+  if (S.max_p <= 32) {
+    auto kern = S.max_p <= 8 ? k_draw_reg<8> : S.max_p <= 16 ? k_draw_reg<16>
+                : S.max_p <= 24 ? k_draw_reg<24> : k_draw_reg<32>;
+    hipLaunchKernelGGL(kern, grid_nm(N, S.m), dim3(TB), 0, st, N, S.m, S.max_p, S.p, S.off,
+                       S.P, m_t, U, n_t, s_t, seed, epoch, lam, theta);
+    return;
+  }
   const size_t lds = (size_t)S.max_p * TS * sizeof(double);
   hipLaunchKernelGGL(k_draw, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
                      S.max_p, S.p, S.off, S.P, m_t, U, n_t, s_t, seed, epoch,
