@@ -225,6 +225,74 @@ __device__ void evolve_one(long i, int j, int m, int max_p, const unsigned *p, c
   }
 }
 
+/* k_evolve with omega and the evolved coefficients in registers
+ * (max_p <= PMAX): the same operations in the same order as evolve_one
+ * (G theta accumulated over n for every k, then chol(W)^T omega), with the
+ * old coefficients read once each from the draw's theta row. */
+template <int PMAX>
+__global__ __launch_bounds__(TB) void k_evolve_reg(
+    long N, int m, int max_p, const unsigned *p, const int *off, long P,
+    const double *U_W, const double *r_t, const double *c_t,
+    const double *beta, const double *G, const double *m_t, uint64_t seed,
+    uint32_t epoch, double *lam, double *theta) {
+  const int j = blockIdx.y;
+  const int pj = (int)p[j];
+  const double *U = U_W + (size_t)j * max_p * max_p;
+  const double *Gj = G ? G + (size_t)j * max_p * max_p : nullptr;
+  const double *mj = m_t + (size_t)j * max_p;
+  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
+    double *th = theta + (size_t)i * P + off[j];
+    double *lp = lam + (size_t)i * m + j;
+    const double eta = sg_beta_eta(r_t[j], beta[j], seed, epoch, (uint32_t)i, (uint32_t)j);
+    const double l = *lp * eta;
+    *lp = l;
+    const double sc = __builtin_sqrt(c_t[j] * l);
+    double om[PMAX], nt[PMAX];
+#pragma unroll
+    for (int q = 0; q < PMAX; q += 2) {
+      if (q < pj) {
+        double z0, z1;
+        sg_normal_pair(sg_rng_block(seed, epoch, SG_PURPOSE_OMEGA, (uint32_t)i,
+                                    (uint32_t)j, (uint32_t)(q >> 1)),
+                       &z0, &z1);
+        om[q] = z0 / sc;
+        if (q + 1 < PMAX)
+          om[q + 1] = (q + 1 < pj) ? z1 / sc : 0.0;
+      }
+    }
+    if (Gj != nullptr) {
+#pragma unroll
+      for (int k = 0; k < PMAX; ++k)
+        nt[k] = 0.0;
+#pragma unroll
+      for (int n = 0; n < PMAX; ++n) {
+        if (n < max_p) {
+          const double v = (n < pj) ? th[n] : mj[n];
+#pragma unroll
+          for (int k = 0; k < PMAX; ++k)
+            if (k < pj)
+              nt[k] += Gj[k + n * max_p] * v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PMAX; ++k)
+        if (k < pj)
+          nt[k] = th[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+      if (k < pj) {
+        double acc = 0.0;
+#pragma unroll
+        for (int n = 0; n <= k; ++n)
+          acc = acc + U[k * max_p + n] * om[n];
+        th[k] = acc + nt[k];
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(TB) void k_evolve(
     long N, int m, int max_p, const unsigned *p, const int *off, long P,
     const double *U_W, const double *r_t, const double *c_t,
@@ -292,6 +360,13 @@ void launch_evolve(const Structure &S, long N, const double *U_W,
                    const double *G, const double *m_t, uint64_t seed,
                    uint32_t epoch, double *lam, double *theta,
                    hipStream_t st) {
+  if (S.max_p <= 32) {
+    auto kern = S.max_p <= 8 ? k_evolve_reg<8> : S.max_p <= 16 ? k_evolve_reg<16>
+                : S.max_p <= 24 ? k_evolve_reg<24> : k_evolve_reg<32>;
+    hipLaunchKernelGGL(kern, grid_nm(N, S.m), dim3(TB), 0, st, N, S.m, S.max_p, S.p, S.off,
+                       S.P, U_W, r_t, c_t, beta, G, m_t, seed, epoch, lam, theta);
+    return;
+  }
   const size_t lds = 2 * (size_t)S.max_p * TB * sizeof(double);
   hipLaunchKernelGGL(k_evolve, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
                      S.max_p, S.p, S.off, S.P, U_W, r_t, c_t, beta, G, m_t,
