@@ -82,6 +82,25 @@ def test_multistep_forecast_evolution(hiplib):
     assert relerr(rg["y"], ro["y"]) <= 1e-9
 
 
+@pytest.mark.parametrize("kmax", [10, 28, 36])
+def test_multistep_forecast_kernel_sizes(hiplib, kmax):
+    """Draws and evolution through every accumulator layout: registers for
+    max_p <= 8, 16, 24, 32 (kmax 10 -> max_p 13, kmax 28 -> 31) and LDS for
+    max_p > 32 (kmax 36 -> 39).  Lambda / theta bit for bit, y to 1e-9."""
+    mod = random_model(50, kmax, T=40, seed=300 + kmax)
+    print(f"multistep forecast m=50 max_p={mod.max_p}", flush=True)
+    g, o = make_pair(mod)
+    filter_steps(g, mod, 1, 30)
+    filter_steps(o, mod, 1, 30)
+    Fk = np.stack([mod.F_forecast(30 + k) for k in range(3)], axis=-1)
+    rg = g.computeForecastDebug(600, 600, Fk)
+    ro = o.computeForecastDebug(600, 600, Fk)
+    assert np.array_equal(rg["lambdas"], ro["lambdas"])
+    assert np.array_equal(rg["thetas"], ro["thetas"])
+    assert relerr(rg["y"], ro["y"]) <= 1e-9
+    g.close()
+
+
 def _vb_pair(mod, prec, N, steps):
     g, o = make_pair(mod, prec)
     filter_steps(g, mod, 1, steps)
