@@ -8,11 +8,13 @@
  *     reduction tree, deterministic);
  *   meanScalarAndLog / compute_mean_m_t / compute_VB_vector1 /
  *     meanBatchedVecVecT (serial-over-N per-thread loops,
- *     kernel_functions.cuh:928-982,1102-1133), by two chunked passes over
- *     the packed draws: pass 1 sums w*lambda, w*log(lambda), w*lambda*theta;
- *     pass 2 forms V = sum w lambda (theta - m)(theta - m)' as a
- *     p x N . N x p contraction on FP64 MFMA (v_mfma_f64_16x16x4f64, the
- *     4 draws of a k-step on the contraction axis);
+ *     kernel_functions.cuh:928-982,1102-1133), by one chunked pass over
+ *     the packed draws: sum w*log(lambda) and, shifted by the naive mean c,
+ *     sum w lambda (theta - c)(theta - c)' with sum w lambda (theta - c)
+ *     and sum w lambda as an extra row / column, as a p x N . N x p
+ *     contraction on FP64 MFMA (v_mfma_f64_16x16x4f64, the 4 draws of a
+ *     k-step on the contraction axis); the final kernel forms the means and
+ *     V = sum w lambda (theta - mbar)(theta - mbar)' from them;
  *   getrf/getri(max_p) + the two m*N gemmBatched calls for Q and
  *     compute_mean_Q_t (SGDLM.cu:404-465), by a Gauss-Jordan inverse of the
  *     p x p block and Qbar = trace(V^{-1} V), which equals
