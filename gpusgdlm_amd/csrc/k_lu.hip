@@ -143,6 +143,7 @@ struct LuArgs {
       e_o_bcbase;
   unsigned e_zero_pair;
   int e_pre_base, e_slot_bytes, e_tab_persist;
+  int e_flag_off; /* LDS byte offset of the early-exit flag (fast kernel) */
   const int *e_slot_init, *e_tab, *e_sg_off, *e_sg_idx, *e_st_off;
   const unsigned *e_st_pairs;
   /* forecast solves through the prefix (fast kernel, SOLVE) */
@@ -181,7 +182,7 @@ __device__ void assemble(T *A, int M, const double *th, const LuArgs &a, int tid
 template <typename T>
 __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const LuArgs &a,
                                 int tid, int nthreads, bool load_tab, int &bad, T &amax,
-                                double &ls) {
+                                double &ls, int *eflag) {
   constexpr unsigned LO = 0xffffu;
   const int NWV = nthreads >> 6; /* wavefronts of the workgroup */
   constexpr int UB = 4;   /* Schur blocks in flight per wavefront */
@@ -250,6 +251,8 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
       const T l = sl[x & LO] / d;
       sl[x & LO] = l;
       amax = fmax(amax, fabs(l));
+      if (bad || !(fabs(l) <= (T)kMaxMultiplier))
+        *eflag = 1;
     }
     __syncthreads();
     const int t1 = tptr[w + 1];
@@ -268,8 +271,10 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   const int *piv = tb + a.e_o_piv;
   for (int q = tid; q < a.e_n1; q += nthreads) {
     const T p = sl[piv[q]];
-    if (!(fabs(p) > (T)0))
+    if (!(fabs(p) > (T)0)) {
       bad = 1;
+      *eflag = 1;
+    }
     ls += log(fabs((double)p));
   }
   for (int r = tid; r < M; r += nthreads)
@@ -632,8 +637,16 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     int bad = 0;
     T amax = (T)0;
     double ls = 0.0;
+    /* early exit: any thread that sees |l| > kMaxMultiplier or a zero
+     * pivot sets the LDS flag; it is read right after barriers, so every
+     * thread takes the same branch, and a flagged draw skips the rest of
+     * its factorization (the partial-pivoting kernel redoes it) */
+    int *eflag = reinterpret_cast<int *>(smem + a.e_flag_off);
+    if (tid == 0)
+      *eflag = 0;
     prefix_assemble<T>(A, M, smem, a.theta + (size_t)d * a.P, a, tid, NT,
-                       !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls);
+                       !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls, eflag);
+    bool stop = *eflag != 0;
 
     /* ---- phase A: U12 = L11^{-1} A12 for the column tiles
      * [c_first, c_first + 16 ntc), L11 = the nqp 16-row blocks of the panel
@@ -823,7 +836,7 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
       }
     };
 
-    for (int K = 0; K < M; K += NB) {
+    for (int K = 0; K < M && !stop; K += NB) {
       const int jb = min(NB, M - K);
       const int nq = jb >> 4;
 
@@ -913,6 +926,11 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
           }
         }
 
+        /* a multiplier beyond the bound or a zero pivot: flag the draw
+         * (read after the barrier below) */
+        if (bad || !(amax <= (T)kMaxMultiplier))
+          *eflag = 1;
+
         /* ---- sub-panel to LDS (Pn rows relative to K; bf16 mode: rows
          * >= jb of the whole panel as -L21 hi/lo, single sub-panel only) */
 #pragma unroll
@@ -950,6 +968,10 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
           }
         }
         __syncthreads();
+        }
+        if (*eflag) {
+          stop = true;
+          break;
         }
         /* trailing update: the next sub-panel's columns, or everything right
          * of the panel after its last sub-panel */
@@ -1286,7 +1308,9 @@ size_t fast_lds_bytes(int M, int NB, int MODE, size_t ts) {
   size_t b = (size_t)M * panel_stride(NB, MODE) * ts + (size_t)NB * 16 * ts + 2 * (size_t)NB * ts +
              (size_t)M * ts;
   b = (b + 15) & ~(size_t)15;
-  return b + 8 * sizeof(double) + 8 * sizeof(float);
+  /* reductions (8 doubles + 8 floats), then 16 bytes for the early-exit
+   * flag at the end */
+  return b + 8 * sizeof(double) + 8 * sizeof(float) + 16;
 }
 
 size_t pivot_lds_bytes(int M, int NB, size_t ts, bool solve) {
@@ -1438,6 +1462,7 @@ void set_fast_layout(LuArgs &a, const Structure &S, const LuPlan &plan) {
   a.e_tab_persist = plan.tab_persist;
   a.e_slot_bytes = (int)((((size_t)S.e_ns * (plan.prec == SGDLM_PREC_FP64 ? 8 : 4)) + 15) & ~(size_t)15);
   a.e_xs_base = (int)plan.xs_base;
+  a.e_flag_off = (int)plan.flag_off;
 }
 
 } // namespace
@@ -1493,13 +1518,22 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
         /* prefix region after the dense one: tables loaded once per workgroup */
         plan->lds = dense + pre_bytes;
         plan->pre_base = dense;
+        plan->flag_off = dense - 16;
         plan->tab_persist = 1;
-      } else if (std::max(dense, pre_bytes) <= cap) {
-        plan->lds = std::max(dense, pre_bytes);
-        plan->pre_base = 0;
-        plan->tab_persist = 0;
       } else {
-        continue;
+        /* prefix region aliased with the dense one; the early-exit flag
+         * moves past the prefix region when that reaches the flag slot */
+        size_t flag = dense - 16, lds = std::max(dense, pre_bytes);
+        if (pre_bytes > dense - 16) {
+          flag = (pre_bytes + 15) & ~(size_t)15;
+          lds = std::max(dense, flag + 16);
+        }
+        if (lds > cap)
+          continue;
+        plan->lds = lds;
+        plan->pre_base = 0;
+        plan->flag_off = flag;
+        plan->tab_persist = 0;
       }
       plan->nb = nb;
       plan->threads = threads;
@@ -1526,6 +1560,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
           plan->fast_grid = plan->grid * per_cu;
           plan->lds = dense + pre_bytes + xs_bytes;
           plan->pre_base = dense;
+          plan->flag_off = dense - 16;
           plan->tab_persist = 1;
           plan->xs_base = dense + pre_bytes;
         }

@@ -84,6 +84,7 @@ struct LuPlan {
   size_t pre_base = 0;        /* LDS byte offset of the prefix slots/tables */
   int tab_persist = 0;        /* prefix tables stay in LDS across draws     */
   size_t xs_base = 0;         /* LDS offset of the solve vector (solves)    */
+  size_t flag_off = 0;        /* LDS offset of the per-draw early-exit flag */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
   size_t work_bytes = 0;      /* global workspace bytes                      */
 };
