@@ -144,6 +144,10 @@ struct LuArgs {
   unsigned e_zero_pair;
   int e_pre_base, e_slot_bytes, e_tab_persist;
   int e_flag_off; /* LDS byte offset of the early-exit flag (fast kernel) */
+  /* solves with an aliased prefix region: per-workgroup global copy of the
+   * prefix slots (nullptr: slots stay in LDS) */
+  void *e_slot_save;
+  long e_slot_stride;
   const int *e_slot_init, *e_tab, *e_sg_off, *e_sg_idx, *e_st_off;
   const unsigned *e_st_pairs;
   /* forecast solves through the prefix (fast kernel, SOLVE) */
@@ -647,6 +651,18 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     prefix_assemble<T>(A, M, smem, a.theta + (size_t)d * a.P, a, tid, NT,
                        !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls, eflag);
     bool stop = *eflag != 0;
+    if constexpr (SOLVE) {
+      if (a.e_slot_save) {
+        /* the dense LU reuses the prefix slots' LDS: keep them for the
+         * substitutions */
+        T *save = reinterpret_cast<T *>(reinterpret_cast<char *>(a.e_slot_save) +
+                                        (size_t)blockIdx.x * a.e_slot_stride);
+        const T *sl = reinterpret_cast<const T *>(smem + a.e_pre_base);
+        for (int q = tid; q < a.e_ns; q += NT)
+          save[q] = sl[q];
+        __syncthreads();
+      }
+    }
 
     /* ---- phase A: U12 = L11^{-1} A12 for the column tiles
      * [c_first, c_first + 16 ntc), L11 = the nqp 16-row blocks of the panel
@@ -1025,8 +1041,21 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     }
     __syncthreads();
     if constexpr (SOLVE) {
-      if (!flagged)
+      if (!flagged) {
+        if (a.e_slot_save) {
+          /* prefix tables and slots back into LDS (the dense region is free) */
+          int *tb = reinterpret_cast<int *>(smem + a.e_pre_base + a.e_slot_bytes);
+          T *sl = reinterpret_cast<T *>(smem + a.e_pre_base);
+          const T *save = reinterpret_cast<const T *>(reinterpret_cast<char *>(a.e_slot_save) +
+                                                      (size_t)blockIdx.x * a.e_slot_stride);
+          for (int w = tid; w < a.e_tab_words; w += NT)
+            tb[w] = a.e_tab[w];
+          for (int q = tid; q < a.e_ns; q += NT)
+            sl[q] = save[q];
+          __syncthreads();
+        }
         prefix_dense_solve<T, NT>(A, M, smem, a.theta + (size_t)d * a.P, a, d, tid);
+      }
     }
   }
 }
@@ -1339,7 +1368,7 @@ int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
   if (plan.threads == 256) {
     /* two workgroups per CU; rows per thread up to 2 (M <= 512); solves
      * only with 32-wide panels */
-    if constexpr (NBI <= 32 && (!SOLVE || NB == 32)) {
+    if constexpr (NBI <= 32 && (!SOLVE || NB == 32 || sizeof(T) == 4)) {
       if (a.M <= 256)
         return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI, SOLVE>, plan.lds, plan.fast_grid, 256,
                              a, st);
@@ -1463,6 +1492,8 @@ void set_fast_layout(LuArgs &a, const Structure &S, const LuPlan &plan) {
   a.e_slot_bytes = (int)((((size_t)S.e_ns * (plan.prec == SGDLM_PREC_FP64 ? 8 : 4)) + 15) & ~(size_t)15);
   a.e_xs_base = (int)plan.xs_base;
   a.e_flag_off = (int)plan.flag_off;
+  a.e_slot_save = plan.slot_stride ? static_cast<char *>(a.work) + plan.slot_save_off : nullptr;
+  a.e_slot_stride = (long)plan.slot_stride;
 }
 
 } // namespace
@@ -1485,6 +1516,8 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   plan->prec = prec;
   plan->nb = 0;
   plan->pivot_nb = 0;
+  plan->slot_stride = 0;
+  plan->slot_save_off = 0;
   plan->fast_M = M2;
   plan->grid = n_cu > 0 ? n_cu : 256;
   const size_t slot_bytes = ((size_t)S.e_ns * ts + 15) & ~(size_t)15;
@@ -1551,10 +1584,11 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
       const int per_cu = pass == 0 ? 2 : 1;
       for (int i = 0; i < 2 && !plan->nb; ++i) {
         const int nb = cands[i];
-        if (M2 > 1024 || (threads == 256 && (nb != 32 || M2 > 512)))
+        if (M2 > 1024 || (threads == 256 && M2 > 512))
           continue;
         const size_t dense = fast_lds_bytes(M2, nb, MODE_NATIVE, ts);
-        if (dense + pre_bytes + xs_bytes <= lds_max / per_cu) {
+        if (dense + pre_bytes + xs_bytes <= lds_max / per_cu &&
+            (threads == 512 || nb == 32)) {
           plan->nb = nb;
           plan->threads = threads;
           plan->fast_grid = plan->grid * per_cu;
@@ -1563,6 +1597,21 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
           plan->flag_off = dense - 16;
           plan->tab_persist = 1;
           plan->xs_base = dense + pre_bytes;
+        } else if (threads == 256 && prec == SGDLM_PREC_FP32 &&
+                   ((pre_bytes + 15) & ~(size_t)15) + xs_bytes + 16 <= dense &&
+                   dense <= lds_max / per_cu) {
+          /* the prefix region and x aliased with the dense one, the prefix
+           * slots kept in global memory across the dense LU (64-wide
+           * panels) */
+          plan->nb = nb;
+          plan->threads = threads;
+          plan->fast_grid = plan->grid * per_cu;
+          plan->lds = dense;
+          plan->pre_base = 0;
+          plan->flag_off = dense - 16;
+          plan->tab_persist = 0;
+          plan->xs_base = (pre_bytes + 15) & ~(size_t)15;
+          plan->slot_stride = ((size_t)S.e_ns * ts + 255) & ~(size_t)255;
         }
       }
     }
@@ -1590,6 +1639,10 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     return SGDLM_ERR_ARG;
   plan->work_bytes =
       (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * M * M * ts;
+  if (plan->slot_stride) {
+    plan->slot_save_off = (plan->work_bytes + 255) & ~(size_t)255;
+    plan->work_bytes = plan->slot_save_off + (size_t)plan->fast_grid * plan->slot_stride;
+  }
   return SGDLM_OK;
 }
 

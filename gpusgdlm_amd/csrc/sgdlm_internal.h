@@ -85,6 +85,10 @@ struct LuPlan {
   int tab_persist = 0;        /* prefix tables stay in LDS across draws     */
   size_t xs_base = 0;         /* LDS offset of the solve vector (solves)    */
   size_t flag_off = 0;        /* LDS offset of the per-draw early-exit flag */
+  /* solves with the prefix region aliased by the dense one: the prefix
+   * slots are kept in global memory across the dense LU */
+  size_t slot_save_off = 0;   /* byte offset in the workspace, 0 = unused  */
+  size_t slot_stride = 0;     /* bytes per workgroup                        */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
   size_t work_bytes = 0;      /* global workspace bytes                      */
 };
