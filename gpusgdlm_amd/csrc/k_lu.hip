@@ -1574,10 +1574,12 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     }
   }
   if (solve && prec != SGDLM_PREC_BF16X3) {
-    /* forecast solves: the prefix slots and tables stay in LDS after the
-     * dense region, then x (n1 + M2 doubles); two 256-thread workgroups per
-     * CU (32-wide panels, M2 <= 512) when that fits in half a CU, else one
-     * 512-thread workgroup */
+    /* forecast solves: two 256-thread workgroups per CU (M2 <= 512) when
+     * that fits in half a CU, else one 512-thread workgroup.  Either the
+     * prefix slots and tables stay in LDS after the dense region, then x
+     * (n1 + M2 doubles), or (FP32 with 64-wide panels) both alias the dense
+     * region and the slots wait in the workspace; with FP64 factors the
+     * aliased layout measured slower and is not used */
     const size_t xs_bytes = (((size_t)S.e_n1 + M2) * sizeof(double) + 15) & ~(size_t)15;
     for (int pass = 0; pass < 2 && !plan->nb; ++pass) {
       const int threads = pass == 0 ? 256 : 512;
