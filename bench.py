@@ -40,7 +40,7 @@ def dense_lu_flops(n: int) -> float:
     return 2.0 / 3.0 * n ** 3
 
 
-def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps):
+def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps, fc_check=0):
     from gpusgdlm_amd import SGDLM
     from gpusgdlm_amd.workloads import configure, filter_steps
     g = SGDLM(strict=True)
@@ -60,11 +60,17 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps):
     stage = {s: g.get_timing(s) for s in ("draw", "lu", "weights", "moments")}
     g.set_timing(False)
     res = g.get_vb_result()
+    lu_fb = g.lu_fallback_count()
+    # one more VB step at epoch 1 (the oracle's first VB call) for the error
+    # table, outside the timed region
+    g.set_rng_state(1234, 0)
+    g.vb_device(N, N)
+    chk = g.get_vb_result()
     out = {"ms_per_step": 1e3 * float(np.mean(times)), "median_ms": 1e3 * float(np.median(times)),
            "p99_ms": 1e3 * float(np.percentile(times, 99)),
            "stage_ms": {k: v[0] / max(v[1], 1) for k, v in stage.items()},
-           "ess": res["ess"], "lu_fallback_draws": g.lu_fallback_count(),
-           "lu_config": g.lu_config(), "elimination": g.lu_elimination()}
+           "ess": res["ess"], "lu_fallback_draws": lu_fb,
+           "lu_config": g.lu_config(), "elimination": g.lu_elimination(), "_check": chk}
     # PCIe-inclusive figure through the full API (outputs copied to host)
     t1 = time.perf_counter()
     g.computeVBPosterior(N, N)
@@ -84,22 +90,26 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps):
         g.set_timing(False)
         out["forecast"] = {"nsim": fc_N, "1step_ms": 1e3 * f1, f"{fc_steps}step_ms": 1e3 * fk,
                            "lu_solve_ms_per_step": fl[0] / max(fl[1], 1)}
+        if fc_check:
+            # the first fc_check draws of a 1-step forecast at fixed epochs
+            # (outside the timed region), for the error table
+            g.set_rng_state(1234, 100)
+            out["_fc_check"] = g.computeForecast(fc_N, fc_N, Ff)[:, :fc_check].copy()
     g.close()
     return out
 
 
-def run_instances(K, N, steps, mmin=20, mmax=400, T_filter=250, seed=20250102):
-    """Many independent models (K instances, m ~ U{mmin..mmax}, k_j ~ U{0..20},
-    T_filter naive filter steps each, N draws, FP32 LU): VB steps of every
-    instance per second, through one batched call per step vs one call per
-    instance."""
+def run_instances(K, N, steps, T_filter=250, seed=20250102):
+    """Many independent models (K instances, m log-uniform on [20, 400],
+    k_j ~ U{0..min(20, m-1)}, T_filter naive filter steps each, N draws,
+    FP32 LU): VB steps of every instance per second through one batched
+    call per step, vs one call per instance; per-instance step times
+    (one call each, synchronous) as p50 / p99."""
     from gpusgdlm_amd import SGDLM, batch_vb
-    from gpusgdlm_amd.workloads import configure, filter_steps, random_model
-    rng = np.random.default_rng(seed)
+    from gpusgdlm_amd.workloads import configure, filter_steps, instance_models
+    mods = instance_models(K, T=T_filter + 2, seed=seed)
     gs = []
-    for i in range(K):
-        m = int(rng.integers(mmin, mmax + 1))
-        mod = random_model(m, min(20, m - 1), T=T_filter + 2, seed=seed + i)
+    for mod in mods:
         g = SGDLM(strict=True)
         g.set_precision("fp32")
         configure(g, mod)
@@ -113,17 +123,120 @@ def run_instances(K, N, steps, mmin=20, mmax=400, T_filter=250, seed=20250102):
     tb = time.perf_counter() - t0
     for g in gs:
         g.vb_device(N, N)
+    per = []
     t0 = time.perf_counter()
     for _ in range(steps):
         for g in gs:
+            t1 = time.perf_counter()
             g.vb_device(N, N)
+            per.append(time.perf_counter() - t1)
     ts = time.perf_counter() - t0
     for g in gs:
         g.close()
-    return {"instances": K, "nsim": N, "m_range": [mmin, mmax], "steps": steps,
+    ms = [md.m for md in mods]
+    return {"instances": K, "nsim": N, "m_distribution": "log-uniform on [20, 400]",
+            "m_quantiles": [int(np.percentile(ms, q)) for q in (0, 50, 90, 100)], "steps": steps,
             "batched_instance_vb_steps_per_s": K * steps / tb,
             "sequential_instance_vb_steps_per_s": K * steps / ts,
-            "batched_ms_per_step": 1e3 * tb / steps}
+            "batched_ms_per_step": 1e3 * tb / steps,
+            "instance_step_ms_p50": 1e3 * float(np.percentile(per, 50)),
+            "instance_step_ms_p99": 1e3 * float(np.percentile(per, 99))}
+
+
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": name, "logical_cpus": os.cpu_count(), "available_to_process": avail}
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def vb_errors(res, vo):
+    """Max errors of one GPU VB result against the oracle's (same draws)."""
+    with np.errstate(divide="ignore"):
+        lwo = np.log(vo["IS_weights"])
+    lw = res["log_weights"] - res["log_sum_det"]
+    fin = np.isfinite(lwo)
+    e = {k: relerr(res[k], vo[k]) for k in ("m", "C", "n", "s")}
+    e["ess_rel"] = abs(res["ess"] / vo["ess"] - 1.0)
+    e["log_weight_abs"] = float(np.max(np.abs(lw[fin] - lwo[fin])))
+    return e
+
+
+def oracle_vb(mod, T_filter, N):
+    from oracle import oracle as orc
+    from gpusgdlm_amd.workloads import configure, filter_steps
+    o = orc.OracleSGDLM(seed=1234, epoch=0)
+    configure(o, mod)
+    filter_steps(o, mod, 1, T_filter)
+    o.computePosterior(mod.y[T_filter], mod.F(T_filter))
+    t0 = time.perf_counter()
+    vo = o.computeVBPosterior(N, N, diag=True)
+    return vo, time.perf_counter() - t0
+
+
+def oracle_forecast(mod, T_filter, n, Ff):
+    """The oracle's 1-step forecast of n draws at the epochs the GPU check
+    uses (posterior -> prior, draws at epoch 101, noise at 102)."""
+    from oracle import oracle as orc
+    from gpusgdlm_amd.workloads import configure, filter_steps
+    o = orc.OracleSGDLM(seed=1234, epoch=100)
+    configure(o, mod)
+    filter_steps(o, mod, 1, T_filter)
+    o.computePosterior(mod.y[T_filter], mod.F(T_filter))
+    o.computePrior()
+    return o.computeForecast(n, n, Ff)
+
+
+def forecast_errors(yg, yo):
+    q = [0.05, 0.5, 0.95]
+    return {"y_rel": relerr(yg, yo),
+            "quantiles_5_50_95_rel": relerr(np.quantile(yg, q, axis=1), np.quantile(yo, q, axis=1))}
+
+
+def run_stretch(m, kmax, N, T_filter, precs, fc_check):
+    """Precision stretch: m series, k ~ U{0..kmax}, N draws; per LU variant
+    the VB step time and the max errors of log-weights, ESS, VB m/C/n/s and
+    1-step forecast 5/50/95% quantiles (first fc_check draws) against the
+    FP64 oracle on the same draws."""
+    from gpusgdlm_amd.workloads import random_model
+    mod = random_model(m, kmax, T=T_filter + 2, seed=20250103)
+    out = {"m": m, "max_p": mod.max_p, "nsim": N, "variants": {}}
+    res = {}
+    for prec in precs:
+        log(f"[m={m} {prec}] VB steps, N={N}")
+        r = run_gpu(mod, prec, N, 3, 1, T_filter, N if prec != "bf16x3" else 0, 1,
+                    fc_check=fc_check if prec != "bf16x3" else 0)
+        res[prec] = r
+    vo, dt = oracle_vb(mod, T_filter, N)
+    yo = oracle_forecast(mod, T_filter, fc_check, mod.F_forecast(T_filter + 1)) if fc_check else None
+    for prec, r in res.items():
+        e = vb_errors(r["_check"], vo)
+        if yo is not None and "_fc_check" in r:
+            e.update({"forecast_" + k: v for k, v in forecast_errors(r["_fc_check"], yo).items()})
+        out["variants"][prec] = {"ms_per_step": r["ms_per_step"], "vb_steps_per_s": 1e3 / r["ms_per_step"],
+                                 "stage_ms": r["stage_ms"], "lu": r["lu_config"],
+                                 "lu_fallback_draws": r["lu_fallback_draws"],
+                                 "errors_vs_oracle": e}
+        log(f"[m={m} {prec}] {r['ms_per_step']:.2f} ms/step, errors {e}")
+    out["oracle_vb_s"] = dt
+    out["forecast_check_draws"] = fc_check
+    return out
 
 
 def cpu_baseline(mod, T_filter, N_cap, budget_s=12.0):
@@ -190,6 +303,10 @@ def main():
                     help="also time K independent models (m ~ U{20..400}, N=1e4) batched vs one call each "
                          "(0 = skip)")
     ap.add_argument("--instances-nsim", type=int, default=10000)
+    ap.add_argument("--no-stretch", action="store_true",
+                    help="skip the m=1000, k~U{0..30}, N=1e4 precision-stretch leg")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the FP64-oracle error table of the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (counter calibration)")
@@ -217,7 +334,8 @@ def main():
     for prec in a.precisions.split(","):
         log(f"[{prec}] {a.warmup} warm-up + {a.steps} timed VB steps, N={N}")
         fcN = a.forecast_nsim if prec in ("fp32", "fp64") else 0
-        results[prec] = run_gpu(mod, prec, N, a.steps, a.warmup, a.filter_steps, fcN, a.forecast_steps)
+        results[prec] = run_gpu(mod, prec, N, a.steps, a.warmup, a.filter_steps, fcN, a.forecast_steps,
+                                fc_check=0 if a.no_errors else 2000)
         r = results[prec]
         log(f"[{prec}] {r['ms_per_step']:.2f} ms/step (median {r['median_ms']:.2f}, p99 {r['p99_ms']:.2f}) "
             f"stages={ {k: round(v, 2) for k, v in r['stage_ms'].items()} } fallback={r['lu_fallback_draws']} "
@@ -267,6 +385,24 @@ def main():
     }
     if "fp64" in results and head != "fp64":
         line["speedup_vs_fp64"] = results["fp64"]["ms_per_step"] / r["ms_per_step"]
+    if not a.no_errors:
+        log(f"[errors] FP64 oracle VB at m={a.m}, N={N}")
+        vo, dt = oracle_vb(mod, a.filter_steps, N)
+        yo = oracle_forecast(mod, a.filter_steps, 2000, mod.F_forecast(a.filter_steps + 1)) \
+            if a.forecast_nsim else None
+        for p, v in results.items():
+            e = vb_errors(v["_check"], vo)
+            if yo is not None and "_fc_check" in v:
+                e.update({"forecast_" + k: x for k, x in forecast_errors(v["_fc_check"], yo).items()})
+            line["variants"][p]["errors_vs_oracle"] = e
+            log(f"[errors] {p}: {e}")
+        line["errors_note"] = (f"max error vs the FP64 oracle on the same draws: VB m/C/n/s relative to "
+                               f"the array's largest entry, ESS relative, normalised log-weights "
+                               f"absolute (oracle VB {dt:.1f} s); forecasts: first 2000 of "
+                               f"{a.forecast_nsim} 1-step draws")
+    if not a.no_stretch:
+        line["precision_stretch"] = run_stretch(1000, 30, 10000, a.filter_steps, a.precisions.split(","),
+                                                0 if a.no_errors else 1000)
     if a.instances:
         log(f"[instances] {a.instances} models, N={a.instances_nsim}")
         line["many_instances"] = run_instances(a.instances, a.instances_nsim, 3)
@@ -275,12 +411,13 @@ def main():
         cb = cpu_baseline(mod, a.filter_steps, N)
         cpu_steps = 1.0 / (cb["dt"] * N / cb["n"])
         line["cpu_baseline"] = {"value": cpu_steps, "unit": "VB steps/s", "cores": cb["threads"], "kind": "port",
+                                "cpu": host_cpu(),
                                 "sample": f"FP64 oracle (OpenMP) draws + LU log-dets of {cb['n']} of the {N} draws "
                                           f"({cb['dt']:.2f} s), scaled to one VB step",
                                 "is_draws_per_s": cb["n"] / cb["dt"],
                                 "one_core_vb_steps_per_s": 1.0 / (cb["dt1"] * N / cb["n1"]),
                                 "one_core_sample": f"{cb['n1']} draws in {cb['dt1']:.2f} s on 1 thread"}
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line, default=float), flush=True)
 
 
 if __name__ == "__main__":

@@ -254,8 +254,12 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
       }
       const T l = sl[x & LO] / d;
       sl[x & LO] = l;
+      /* a multiplier beyond the bound, or NaN (which fmax would drop),
+       * flags the draw through bad as well as the early-exit flag */
+      if (!(fabs(l) <= (T)kMaxMultiplier))
+        bad = 1;
       amax = fmax(amax, fabs(l));
-      if (bad || !(fabs(l) <= (T)kMaxMultiplier))
+      if (bad)
         *eflag = 1;
     }
     __syncthreads();
@@ -929,6 +933,8 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
                   l = v[rr][j] / dj;
                 v[rr][j] = l;
                 amax = fmax(amax, fabs(l));
+                if (__builtin_isnan(l))
+                  bad = 1; /* fmax drops NaN */
 #pragma unroll
                 for (int c4 = c4s; c4 < NBI / VW; ++c4) {
                   const VV x = reinterpret_cast<const VV *>(pr)[c4];

@@ -51,6 +51,9 @@ struct sgdlm_ctx {
   int n_cu = 0;
   size_t lds_max = 65536;
   hipStream_t stream = nullptr;
+  /* pinned host copy of the last VB call's weight statistics (VbBuffers
+   * stats: max log-det, sum exp(l - max), sum w^2, zero weights) */
+  double *h_stats = nullptr;
 
   size_t m = 0, max_p = 0;
   bool memory_initialized = false;
@@ -214,6 +217,18 @@ int check_prior(sgdlm_ctx *c, bool want_prior) {
 int check_evo(sgdlm_ctx *c) {
   if (!c->use_state_evolution_matrix)
     return fail(c, SGDLM_ERR_STATE, "state evolution matrix not enabled");
+  return SGDLM_OK;
+}
+
+/* The Philox counter holds the epoch in 24 bits (epoch << 8 | purpose):
+ * a call that would take it to 2^24 fails instead of silently repeating
+ * the random streams of earlier calls (sgdlm_set_rng_state starts a new
+ * sequence). */
+int check_epochs(sgdlm_ctx *c, uint32_t n) {
+  if ((uint64_t)c->epoch + n >= (1ull << 24))
+    return fail(c, SGDLM_ERR_STATE,
+                "Philox epoch counter exhausted (epoch %u + %u >= 2^24); reset it with "
+                "sgdlm_set_rng_state", c->epoch, n);
   return SGDLM_OK;
 }
 
@@ -433,6 +448,8 @@ int run_vb(sgdlm_ctx *c) {
   if ((rc = ensure_sim_buffers(c)))
     return rc;
   const long N = (long)c->nsim;
+  if ((rc = check_epochs(c, 1)))
+    return rc;
   const uint32_t ep = ++c->epoch;
   {
     StageTimer t(c, "draw");
@@ -466,7 +483,19 @@ int run_vb(sgdlm_ctx *c) {
                    M_<double>(c, "m"), M_<double>(c, "n"), M_<double>(c, "s"), B, c->stream);
   }
   CHECK_LAUNCH(c);
+  HIPCHK(c, hipMemcpyAsync(c->h_stats, B.stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
   c->vb_valid = true;
+  return SGDLM_OK;
+}
+
+/* After the stream of a VB call has drained: every importance weight zero
+ * (each |det(I - Gamma_i)| zero or non-finite) is an error, also when the
+ * results stay on the device. */
+int vb_status(sgdlm_ctx *c) {
+  if (!(c->h_stats[1] > 0.0))
+    return fail(c, SGDLM_ERR_NUMERIC,
+                "every importance weight is zero (all |det(I-Gamma)| zero or non-finite)");
   return SGDLM_OK;
 }
 
@@ -508,6 +537,12 @@ int sgdlm_create(sgdlm_ctx **out, size_t no_gpus) {
     delete c;
     return SGDLM_ERR_DEVICE;
   }
+  if (hipHostMalloc((void **)&c->h_stats, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SGDLM_ERR_ALLOC;
+  }
   *out = c;
   return SGDLM_OK;
 }
@@ -523,6 +558,7 @@ int sgdlm_destroy(sgdlm_ctx *c) {
     (void)hipEventDestroy(pe.b);
   }
   (void)hipStreamDestroy(c->stream);
+  (void)hipHostFree(c->h_stats);
   delete c;
   return SGDLM_OK;
 }
@@ -914,19 +950,16 @@ int sgdlm_compute_vb_posterior(sgdlm_ctx *c, double *mean_m, double *mean_C,
   if ((rc = run_vb(c)))
     return rc;
   const bool any = mean_m || mean_C || mean_n || mean_s || w || sum_det;
-  if (!any)
-    return sync_and_collect(c);
-  double stats[8];
-  rc = sgdlm_get_vb_result(c, mean_m, mean_C, mean_n, mean_s, w, nullptr, nullptr, nullptr);
-  if (rc)
-    return rc;
-  if ((rc = d2h(c, stats, SIM<double>(c, "stats"), 4 * 8)) || (rc = sync_and_collect(c)))
+  if (any) {
+    rc = sgdlm_get_vb_result(c, mean_m, mean_C, mean_n, mean_s, w, nullptr, nullptr, nullptr);
+    if (rc)
+      return rc;
+  }
+  if ((rc = sync_and_collect(c)))
     return rc;
   if (sum_det)
-    *sum_det = std::exp(stats[0]) * stats[1];
-  if (!(stats[1] > 0.0))
-    return fail(c, SGDLM_ERR_NUMERIC, "every importance weight is zero (all |det(I-Gamma)| zero or non-finite)");
-  return SGDLM_OK;
+    *sum_det = std::exp(c->h_stats[0]) * c->h_stats[1];
+  return vb_status(c);
 }
 
 int sgdlm_batch_compute_vb_posterior(sgdlm_ctx *const *ctxs, size_t count, int *status) {
@@ -948,6 +981,8 @@ int sgdlm_batch_compute_vb_posterior(sgdlm_ctx *const *ctxs, size_t count, int *
   for (size_t i = 0; i < count; ++i) {
     if (!rcs[i])
       rcs[i] = sync_and_collect(ctxs[i]);
+    if (!rcs[i])
+      rcs[i] = vb_status(ctxs[i]);
     if (status)
       status[i] = rcs[i];
     if (rcs[i] && !first)
@@ -1002,6 +1037,8 @@ int sgdlm_evolve_forecast_samples(sgdlm_ctx *c, const double *G) {
     return rc;
   if ((rc = ensure_sim_buffers(c)))
     return rc;
+  if ((rc = check_epochs(c, 1)))
+    return rc;
   if (G && (rc = h2d(c, M_<double>(c, "G"), G, c->m * c->max_p * c->max_p * 8)))
     return rc;
   const uint32_t ep = ++c->epoch;
@@ -1028,6 +1065,8 @@ int sgdlm_compute_forecast(sgdlm_ctx *c, double *y, const double *x,
     return rc;
   if (!x)
     return fail(c, SGDLM_ERR_ARG, "null x_tp1");
+  if ((rc = check_epochs(c, use_existing ? 1 : 2)))
+    return rc;
   if ((rc = ensure_sim_buffers(c)))
     return rc;
   const size_t m = c->m, mp = c->max_p, N = c->nsim;
@@ -1166,6 +1205,31 @@ int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, in
     *nb = plan.nb;
   if (threads)
     *threads = plan.nb ? plan.threads : 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_get_lu_solve_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, int *threads,
+                              int *slots_in_workspace) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (!c->parents_set)
+    return SGDLM_ERR_STATE;
+  const int prec = (c->prec == SGDLM_PREC_FP64) ? SGDLM_PREC_FP64 : SGDLM_PREC_FP32;
+  LuPlan plan;
+  int rc = lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, &plan);
+  if (rc)
+    return fail(c, rc, "no LU solve configuration for m=%zu", c->m);
+  if (grid)
+    *grid = plan.nb ? plan.fast_grid : plan.pivot_grid;
+  if (lds)
+    *lds = plan.nb ? plan.lds : plan.pivot_lds;
+  if (nb)
+    *nb = plan.nb;
+  if (threads)
+    *threads = plan.nb ? plan.threads : 0;
+  if (slots_in_workspace)
+    *slots_in_workspace = plan.slot_stride != 0;
   return SGDLM_OK;
 }
 

@@ -369,6 +369,15 @@ class SGDLM:
         self._call("sgdlm_get_lu_config", C.byref(g), C.byref(l), C.byref(nb), C.byref(th))
         return {"grid": g.value, "lds": l.value, "nb": nb.value, "threads": th.value}
 
+    def lu_solve_config(self):
+        """Configuration of the forecast-solve LU kernels at the current
+        precision (nb = 0: partial pivoting for every draw)."""
+        g, l, nb, th, sw = C.c_int(), C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
+        self._call("sgdlm_get_lu_solve_config", C.byref(g), C.byref(l), C.byref(nb), C.byref(th),
+                   C.byref(sw))
+        return {"grid": g.value, "lds": l.value, "nb": nb.value, "threads": th.value,
+                "slots_in_workspace": bool(sw.value)}
+
     def set_lu_tuning(self, threads: int = 0, nb: int = 0):
         """Pin the fast LU kernel's workgroup size (256 / 512) and panel width
         (16 / 32 / 64); 0 = automatic."""

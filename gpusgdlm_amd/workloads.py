@@ -162,3 +162,19 @@ def filter_steps(model_api, mod: Model, t0: int, t1: int):
     for t in range(t0, t1):
         model_api.computePosterior(mod.y[t], mod.F(t))
         model_api.computePrior()
+
+
+def instance_sizes(K: int, seed: int = 20250102, mmin: int = 20, mmax: int = 400) -> list:
+    """Series counts of K independent models, log-uniform on [mmin, mmax]
+    (m = round(exp(U(log mmin, log mmax))))."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(mmin), np.log(mmax), size=K)
+    return [int(min(mmax, max(mmin, round(float(np.exp(x)))))) for x in u]
+
+
+def instance_models(K: int, T: int, seed: int = 20250102, mmin: int = 20, mmax: int = 400,
+                    kmax: int = 20) -> list:
+    """K independent random models (each with its own data seed), m
+    log-uniform on [mmin, mmax], k_j ~ U{0..min(kmax, m - 1)} parents."""
+    return [random_model(m, min(kmax, m - 1), T=T, seed=seed + 1 + i)
+            for i, m in enumerate(instance_sizes(K, seed, mmin, mmax))]

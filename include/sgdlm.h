@@ -172,6 +172,14 @@ int sgdlm_get_lu_fallback_count(const sgdlm_ctx *ctx, long *count);
  * through the partial-pivoting kernel). */
 int sgdlm_get_lu_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
                         int *nb, int *threads);
+/* The same for the forecast solves (computeForecast) at the current
+ * precision (FP64 factors for SGDLM_PREC_FP64, else FP32 factors with one
+ * FP64-residual refinement): nb = 0 means every draw is solved by the
+ * partial-pivoting kernel; slots_in_workspace = 1 when the sparse prefix's
+ * LDS slots wait in the global workspace during the dense LU (their LDS is
+ * shared with the dense panel). */
+int sgdlm_get_lu_solve_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
+                              int *nb, int *threads, int *slots_in_workspace);
 /* Pins the fast VB LU kernel's workgroup size (256: two workgroups per CU,
  * 512: one) and panel width (16, 32, 64); 0 = automatic.  A configuration
  * that does not fit makes the next VB call fail with SGDLM_ERR_ARG. */

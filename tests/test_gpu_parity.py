@@ -345,3 +345,68 @@ def test_forecast_m400_fast_path(hiplib, prec, tol):
     yo = o.computeForecast(N, N, mod.F_forecast(50))
     assert g.lu_fallback_count() <= N // 8
     assert relerr(yg, yo) <= tol
+
+
+def _dense_I_minus_Gamma(mod, th):
+    A = np.eye(mod.m)
+    spf = mod.sp_u32.ravel(order="F")
+    thf = th.ravel(order="F")
+    for q, v in enumerate(spf):
+        if v < mod.m * mod.m:
+            A[v % mod.m, v // mod.m] = -thf[q]
+    return A
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_forecast_m400_pivoting_fallback(hiplib, prec):
+    """Forecast solves whose draws need row interchanges: a seventh of the
+    simultaneous coefficients of an m = 400 model get prior means near
+    +-2.5, so most draws leave the no-interchange fast path (multiplier
+    above 2) and are solved by the partial-pivoting kernel (L and U kept,
+    FP32: one FP64-residual refinement).  These I - Gamma have condition
+    numbers kappa from ~1e3 to ~1e7, so the bar is per draw and
+    condition-aware: relative error <= 1e-12 (FP64) / 1e-8 (FP32), or else
+    <= 1e-15 kappa (FP64, backward-stable LU) / (kappa u32)^2 with
+    u32 = 6e-8 (FP32: one refinement step squares the unrefined error
+    ~kappa u32; a missing refinement fails it for every kappa < 1e7).  The
+    lambda / theta draws themselves are bit-exact."""
+    mod = random_model(400, 20, T=60, seed=13)
+    g, o = make_pair(mod, prec)
+    filter_steps(g, mod, 1, 50)
+    filter_steps(o, mod, 1, 50)
+    g.computePosterior(mod.y[50], mod.F(50))
+    o.computePosterior(mod.y[50], mod.F(50))
+    prm = o.getParameters()
+    mt = prm["m"].copy(order="F")
+    m = mod.m
+    for j in range(m):
+        for k in range(mod.max_p):
+            if mod.sp_u32[k, j] < m * m and (j + k) % 7 == 0:
+                mt[k, j] = 2.5 if (j + k) % 2 else -2.3
+    for api in (g, o):
+        api.setPosteriorParameters(mt, prm["C"] * 0.01, prm["n"], prm["s"])
+        api.computePrior()
+    cfg = g.lu_solve_config()
+    print(f"forecast fallback m=400 {prec}: solve plan {cfg}", flush=True)
+    assert cfg["nb"] > 0
+    if prec == "fp32":
+        assert cfg["threads"] == 256 and cfg["nb"] == 64 and cfg["slots_in_workspace"]
+    N = 512
+    rg = g.computeForecastDebug(N, N, mod.F_forecast(51))
+    fb = g.lu_fallback_count()
+    ro = o.computeForecastDebug(N, N, mod.F_forecast(51))
+    assert np.array_equal(rg["thetas"], ro["thetas"]) and np.array_equal(rg["lambdas"], ro["lambdas"])
+    yg, yo = rg["y"], ro["y"]
+    err = np.linalg.norm(yg - yo, axis=0) / np.linalg.norm(yo, axis=0)
+    easy = 1e-12 if prec == "fp64" else 1e-8
+    hard = np.nonzero(~(err <= easy))[0]
+    print(f"  fallback {fb} of {N}; per-draw rel err median {np.median(err):.2e} max {err.max():.2e}; "
+          f"{len(hard)} draws above {easy:g}", flush=True)
+    assert fb >= N // 4
+    assert len(hard) <= N // 10
+    for i in hard:
+        kappa = np.linalg.cond(_dense_I_minus_Gamma(mod, rg["thetas"][:, :, i]))
+        bound = 1e-15 * kappa if prec == "fp64" else (kappa * 6e-8) ** 2
+        print(f"  draw {i}: kappa {kappa:.2e}, err {err[i]:.2e}, bound {bound:.2e}", flush=True)
+        assert err[i] <= bound, (i, kappa, err[i])
+    g.close()

@@ -287,3 +287,29 @@ def test_evolution_moments(oracle_lib):
     z = th[:P, 0, :] * np.sqrt(0.5 * lam[0])
     assert np.allclose(np.cov(z), Wb, rtol=0.03, atol=0.03 * np.max(np.abs(Wb)))
     assert np.all(th[P:, 0, :] == 0)
+
+
+def test_evolution_with_G_is_G_theta(oracle_lib):
+    """sgo_evolve with G (SGDLM.cu:729-740): beta = 1 (eta = 1) and W = 0
+    (no noise) leave exactly theta' = G_j theta on every series, G_j read
+    column-major ([j*max_p^2 + r + c*max_p]) -- a transposed G, a skipped
+    column or a dropped padding row fails against numpy's product."""
+    rng = np.random.default_rng(8)
+    m, mp, N = 3, 4, 50
+    p = np.array([4, 2, 3], dtype=np.uint32)
+    G = np.zeros((mp, mp, m), order="F")
+    for j in range(m):
+        G[:, :, j] = rng.normal(size=(mp, mp))
+    W = np.zeros((mp, mp, m), order="F")
+    lam0 = rng.uniform(0.5, 2.0, size=(m, N))
+    th0 = rng.normal(size=(mp, m, N))
+    lam, th = np.asfortranarray(lam0.copy()), np.asfortranarray(th0.copy())
+    r_t, c_t, beta = np.full(m, 10.0), np.full(m, 0.5), np.ones(m)
+    orc.lib().sgo_evolve(m, mp, orc.u(p), orc.d(W), orc.d(r_t), orc.d(c_t), orc.d(beta),
+                         orc.d(G), N, 11, 3, orc.d(lam), orc.d(th))
+    assert np.array_equal(lam, lam0)
+    for j in range(m):
+        want = G[:, :, j] @ th0[:, j, :]
+        assert np.allclose(th[:, j, :], want, rtol=1e-14, atol=1e-14), j
+    # the transpose would differ (G_j is not symmetric)
+    assert not np.allclose(G[:, :, 0].T @ th0[:, 0, :], th[:, 0, :])
