@@ -59,6 +59,7 @@
 #include <algorithm>
 #include <math.h>
 
+#include "lu_prefix.h"
 #include "lu_symbolic.h"
 #include "sgdlm_internal.h"
 
@@ -111,13 +112,6 @@ __device__ __forceinline__ float bf2f(unsigned short h) {
 }
 
 enum { MODE_NATIVE = 0, MODE_BF16X3 = 1 };
-
-/* Threshold partial pivoting: the diagonal pivot of the fast kernel is
- * accepted while every multiplier satisfies |l| <= 1 / tau with tau = 1/2
- * (LAPACK's partial pivoting is tau = 1); element growth per step stays
- * below 1 + 1/tau = 3.  Draws that exceed it are refactored with partial
- * pivoting by k_lu_pivot. */
-constexpr float kMaxMultiplier = 2.0f;
 
 /* LDS row stride of the panel: odd for conflict-free column access; 16-byte
  * aligned rows for the bf16 operand loads. */
@@ -191,100 +185,21 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   const int NWV = nthreads >> 6; /* wavefronts of the workgroup */
   constexpr int UB = 4;   /* Schur blocks in flight per wavefront */
   constexpr int RR = 8;   /* rounds loaded ahead per block */
+  constexpr int GB = 8;   /* global reads in flight per thread */
   const int lane = tid & 63, wave = tid >> 6;
   T *sl = reinterpret_cast<T *>(smem + a.e_pre_base);
   int *tb = reinterpret_cast<int *>(smem + a.e_pre_base + a.e_slot_bytes);
-  /* the global reads below are issued GB at a time per thread: one
-   * round trip per batch instead of one per element */
-  constexpr int GB = 8;
-  if (load_tab) {
-    for (int w0 = tid; w0 < a.e_tab_words; w0 += GB * nthreads) {
-      int x[GB];
-#pragma unroll
-      for (int u = 0; u < GB; ++u) {
-        const int w = w0 + u * nthreads;
-        x[u] = w < a.e_tab_words ? a.e_tab[w] : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < GB; ++u) {
-        const int w = w0 + u * nthreads;
-        if (w < a.e_tab_words)
-          tb[w] = x[u];
-      }
-    }
-  }
-  for (int s0 = tid; s0 < a.e_ns; s0 += GB * nthreads) {
-    int code[GB];
-#pragma unroll
-    for (int u = 0; u < GB; ++u) {
-      const int s = s0 + u * nthreads;
-      code[u] = s < a.e_ns ? a.e_slot_init[s] : SLOT_ZERO;
-    }
-    T x[GB];
-#pragma unroll
-    for (int u = 0; u < GB; ++u) {
-      const int v = code[u];
-      x[u] = v >= 0 ? (T)(-th[v]) : (v == SLOT_ONE ? (T)1 : (v == SLOT_MINUS_ONE ? (T)-1 : (T)0));
-    }
-#pragma unroll
-    for (int u = 0; u < GB; ++u) {
-      const int s = s0 + u * nthreads;
-      if (s < a.e_ns)
-        sl[s] = x[u];
-    }
-  }
+  const PrefixTabs pt{a.e_ns, a.e_waves, a.e_n1, a.e_tab_words, a.e_o_piv, a.e_o_mptr,
+                      a.e_o_mult, a.e_o_tptr, a.e_o_tslot, a.e_o_cptr, a.e_o_contrib,
+                      a.e_slot_init, a.e_tab};
+  /* zero fill of the workspace, ordered before the writes below by the
+   * prefix elimination's barriers */
   const size_t nvec = (size_t)M * M * sizeof(T) / 16;
   float4 *A4 = reinterpret_cast<float4 *>(A);
   const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
   for (size_t q = tid; q < nvec; q += nthreads)
     A4[q] = z;
-  __syncthreads();
-  const int *mptr = tb + a.e_o_mptr, *tptr = tb + a.e_o_tptr, *cptr = tb + a.e_o_cptr;
-  const unsigned *mult = reinterpret_cast<const unsigned *>(tb + a.e_o_mult);
-  const unsigned *contrib = reinterpret_cast<const unsigned *>(tb + a.e_o_contrib);
-  const int *tslot = tb + a.e_o_tslot;
-  for (int w = 0; w < a.e_waves; ++w) {
-    const int m1 = mptr[w + 1];
-    for (int it = mptr[w] + tid; it < m1; it += nthreads) {
-      const unsigned x = mult[it];
-      T d = sl[x >> 16];
-      if (!(fabs(d) > (T)0)) {
-        bad = 1;
-        d = (T)1;
-      }
-      const T l = sl[x & LO] / d;
-      sl[x & LO] = l;
-      /* a multiplier beyond the bound, or NaN (which fmax would drop),
-       * flags the draw through bad as well as the early-exit flag */
-      if (!(fabs(l) <= (T)kMaxMultiplier))
-        bad = 1;
-      amax = fmax(amax, fabs(l));
-      if (bad)
-        *eflag = 1;
-    }
-    __syncthreads();
-    const int t1 = tptr[w + 1];
-    for (int t = tptr[w] + tid; t < t1; t += nthreads) {
-      const int s = tslot[t];
-      T acc = sl[s];
-      const int e1 = cptr[t + 1];
-      for (int e = cptr[t]; e < e1; ++e) {
-        const unsigned c = contrib[e];
-        acc = fma(-sl[c & LO], sl[c >> 16], acc);
-      }
-      sl[s] = acc;
-    }
-    __syncthreads();
-  }
-  const int *piv = tb + a.e_o_piv;
-  for (int q = tid; q < a.e_n1; q += nthreads) {
-    const T p = sl[piv[q]];
-    if (!(fabs(p) > (T)0)) {
-      bad = 1;
-      *eflag = 1;
-    }
-    ls += log(fabs((double)p));
-  }
+  prefix_eliminate<T>(sl, tb, th, pt, tid, nthreads, load_tab, bad, amax, ls, eflag);
   for (int r = tid; r < M; r += nthreads)
     A[(size_t)r * M + r] = (T)1;
   for (int e0 = tid; e0 < a.e_nsg; e0 += GB * nthreads) {
@@ -1645,6 +1560,16 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   }
   if (plan->pivot_nb == 0)
     return SGDLM_ERR_ARG;
+  /* FP32 / BF16x3 log-determinants of dense blocks up to 288: the
+   * register-resident kernel, unless a fast-kernel shape was pinned */
+  plan->rs_MT = 0;
+  if (!solve && prec != SGDLM_PREC_FP64 && !want_threads && !want_nb && S.e_rb_rows > 0) {
+    plan->rs_MT = M2 / 16;
+    plan->rs_grid = plan->grid;
+    plan->rs_lds = S.e_rb_lds;
+    plan->rs_panel_base = S.e_rb_panel_base;
+    plan->rs_flag_base = S.e_rb_flag_base;
+  }
   plan->work_bytes =
       (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * M * M * ts;
   if (plan->slot_stride) {
@@ -1661,7 +1586,7 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   a.logdet = logdet;
   a.flag_count = flags;
   a.flag_list = flags + 1;
-  if (plan.nb == 0) {
+  if (plan.nb == 0 && plan.rs_MT == 0) {
     /* no fast configuration: partial pivoting for every draw */
     a.flag_count = nullptr;
     a.flag_list = nullptr;
@@ -1671,8 +1596,13 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   }
   if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
     return SGDLM_ERR_DEVICE;
-  set_fast_layout(a, S, plan);
-  int rc = dispatch_fast<false>(a, plan, st);
+  int rc;
+  if (plan.rs_MT > 0) {
+    rc = launch_lu_rs(S, plan, N, theta, logdet, flags, st);
+  } else {
+    set_fast_layout(a, S, plan);
+    rc = dispatch_fast<false>(a, plan, st);
+  }
   if (rc)
     return rc;
   a.M = S.M;

@@ -1,0 +1,610 @@
+/*
+ * k_lu_rs.hip -- log|det(I - Gamma_i)| of every importance draw with the
+ * dense Schur complement held in registers (FP32 and BF16x3 updates),
+ * gfx950.
+ *
+ * Replaces fillGammas + cublasXgetrfBatched + detFromLU of the VB loop
+ * (rSGDLM/src/SGDLM/SGDLM.cu:243-298, kernel_functions.cuh:821-842,904-926)
+ * for dense blocks of up to 288 x 288 after the sparse prefix.
+ *
+ * One 512-thread workgroup per CU, persistent over draws.  Per draw:
+ *   1. the sparse prefix (lu_prefix.h) on LDS slots;
+ *   2. the Schur complement S (16 MT x 16 MT, identity padding) assembled
+ *      through an LDS staging band of BR rows at a time (zero fill, the
+ *      entries of I - Gamma, then every prefix-updated entry as the sum over
+ *      its slot pairs, ELL blocks of 64) and loaded into registers: the 8
+ *      wavefronts form a 2 x 4 grid and wavefront (wr, wc) keeps the 16 x 16
+ *      tiles (I, J) with I = wr (mod 2), J = wc (mod 4) as MFMA accumulators
+ *      (lane l holds rows 4 (l / 16) + r, r = 0..3, of column l % 16);
+ *   3. a right-looking LU over 16-wide block columns k with no row
+ *      interchanges, S never leaving the chip:
+ *        - the owner of tile (k, k) factors it (one row per lane, pivot rows
+ *          broadcast with readlane) and forms L_kk^-1 and U_kk^-1 (column
+ *          substitutions, one column per lane);
+ *        - U_kJ = L_kk^-1 A_kJ by the owners of row k (the accumulator tile is
+ *          the MFMA B operand directly); L_Ik = A_Ik U_kk^-1 for the column
+ *          tiles staged in LDS, spread over the other four wavefronts;
+ *        - A_IJ -= L_Ik U_kJ on MFMA into the accumulators: FP32
+ *          v_mfma_f32_16x16x4f32, or BF16x3 (hi*hi + hi*lo + lo*hi of
+ *          v_mfma_f32_16x16x16bf16_1k, FP32 accumulation);
+ *      the next block column is updated first and its diagonal tile
+ *      factored while the other wavefronts finish the trailing update, so
+ *      every step costs two workgroup barriers;
+ *   4. log|det| = sum of log|a_kk| over the prefix and dense pivots in FP64,
+ *      reduced in a fixed order.
+ * Threshold partial pivoting as in k_lu_fast: a draw with a multiplier
+ * |l| > 2 (diagonal-tile or L_Ik entries), a zero or NaN pivot is appended
+ * to the device list that k_lu_pivot refactors with partial pivoting.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "lu_prefix.h"
+#include "sgdlm_internal.h"
+
+namespace sgdlm {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+enum { RS_FP32 = 0, RS_BF16X3 = 1 };
+
+constexpr int TR = 20;         /* LDS row stride of a staged tile (floats) */
+constexpr int TSF = 16 * TR;   /* floats per staged tile                   */
+
+struct RsArgs {
+  int MT;                      /* tiles per dimension of S                 */
+  long n_draws;
+  const double *theta;
+  long P;
+  double *logdet;
+  int *flag_count, *flag_list;
+  PrefixTabs pre;
+  int pre_base, slot_bytes, tab_persist;
+  int flag_base, red_base;
+  int x_base, l_base, u_base, lu_base, linv_base, uinvt_base, urev_base, piv_base;
+  int stage_base, band_rows, n_band;
+  const int *blk_ptr, *st_off, *brounds, *bcbase, *sg_ptr, *sg_off, *sg_idx;
+  const unsigned *st_pairs;
+  unsigned zero_pair;
+};
+
+__device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma_bf16(s16x4 a, s16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+/* float -> bf16 bits, round to nearest even (finite inputs). */
+__device__ __forceinline__ unsigned short to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short h) {
+  return __uint_as_float((unsigned)h << 16);
+}
+
+/* The two halves of a staged BF16x3 fragment: elements 0-3 hold the bf16
+ * high parts, 4-7 the low parts (explicit shuffles: the ext-vector .lo/.hi
+ * accessors on this vector type were compiled to the same half). */
+__device__ __forceinline__ s16x4 hi_part(u16x8 v) {
+  return __builtin_bit_cast(s16x4, __builtin_shufflevector(v, v, 0, 1, 2, 3));
+}
+__device__ __forceinline__ s16x4 lo_part(u16x8 v) {
+  return __builtin_bit_cast(s16x4, __builtin_shufflevector(v, v, 4, 5, 6, 7));
+}
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+/* LDS written by some lanes of a wavefront and then read by others of the
+ * same wavefront: keep program order (the hardware executes one
+ * wavefront's LDS instructions in order). */
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* LDS pointers (address space 3): the out-of-line diagonal factorization
+ * then uses LDS instructions instead of flat ones. */
+typedef __attribute__((address_space(3))) float lds_f;
+typedef __attribute__((address_space(3))) f32x4 lds_f4;
+
+__device__ __forceinline__ f32x4 lds_load4(const lds_f *p) {
+  return *reinterpret_cast<const lds_f4 *>(p);
+}
+__device__ __forceinline__ void lds_store4(lds_f *p, f32x4 v) {
+  *reinterpret_cast<lds_f4 *>(p) = v;
+}
+
+/* LU without interchanges of the 16 x 16 tile X (LDS, row stride TR) and
+ * its inverse factors: one wavefront.  Writes L^-1 row-major to Linv, U^-1
+ * transposed to UinvT (UinvT[c][i] = U^-1[i][c]) and |u_ii| to piv[i]; LU
+ * and Urev are scratch tiles.  A zero or NaN pivot, or a multiplier beyond
+ * kMaxMultiplier sets bad. */
+__device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *Linv,
+                                 lds_f *UinvT, lds_f *piv, int lane, int &bad) {
+  const int i = lane & 15;
+  float v[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const f32x4 t = lds_load4(X + i * TR + 4 * q);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      v[4 * q + e] = t[e];
+  }
+  /* one row per lane; pivot row j broadcast from lane j with readlane */
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    /* the whole pivot row first: back-to-back readlanes cover each
+     * other's SGPR write-to-read latency */
+    float p[16];
+#pragma unroll
+    for (int c = j; c < 16; ++c)
+      p[c] = lane_value(v[c], j);
+    const float dj = p[j];
+    if (!(fabsf(dj) > 0.f))
+      bad = 1;
+    const float rd = __builtin_amdgcn_rcpf(dj);
+    const bool below = i > j;
+    const float l = below ? v[j] * rd : 0.f;
+    if (!(fabsf(l) <= kMaxMultiplier))
+      bad = 1;
+    v[j] = below ? l : v[j];
+#pragma unroll
+    for (int c = j + 1; c < 16; ++c)
+      v[c] = fmaf(-l, p[c], v[c]);
+  }
+  if (lane < 16) {
+    float u = v[0];
+#pragma unroll
+    for (int c = 1; c < 16; ++c)
+      u = (i == c) ? v[c] : u;
+    piv[i] = fabsf(u);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      lds_store4(LU + i * TR + 4 * q, f32x4{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]});
+      /* Urev[r][t] = U[15 - r][15 - t]: this row reversed into row 15 - i */
+      lds_store4(Urev + (15 - i) * TR + 4 * q,
+                 f32x4{v[15 - 4 * q], v[14 - 4 * q], v[13 - 4 * q], v[12 - 4 * q]});
+    }
+  }
+  wave_lds_sync();
+  /* lanes 0-15: column c of L^-1 (forward substitution with the unit lower
+   * L); lanes 16-31: column c of T^-1 for the lower triangular T = J U J
+   * = Urev (J the reversal), whose inverse is J U^-1 J.  The same
+   * instruction stream on per-lane rows. */
+  const bool upper = (lane & 16) != 0;
+  const lds_f *T = upper ? Urev : LU;
+  float x[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float rw[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const f32x4 t = lds_load4(T + r * TR + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        rw[4 * q + e] = t[e];
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < r; ++t)
+      s = fmaf(rw[t], x[t], s);
+    const float rdg = upper ? __builtin_amdgcn_rcpf(rw[r]) : 1.f;
+    x[r] = (r == i) ? rdg : (r < i ? 0.f : -s * rdg);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      Linv[r * TR + i] = x[r];
+  } else if (lane < 32) {
+    /* U^-1[15 - r][15 - i] = x[r]: row 15 - i of UinvT, reversed */
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      lds_store4(UinvT + (15 - i) * TR + 4 * q,
+                 f32x4{x[15 - 4 * q], x[14 - 4 * q], x[13 - 4 * q], x[12 - 4 * q]});
+  }
+}
+
+template <int WR, int WC, int RT, int CT, int MODE>
+__global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a) {
+  constexpr int NW = WR * WC, NT = 64 * NW;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  constexpr unsigned LO = 0xffffu;
+  constexpr int UB = 4; /* Schur blocks in flight per wavefront */
+  constexpr int RR = 8; /* rounds loaded ahead per block        */
+  const int tid = threadIdx.x, lane = tid & 63;
+  /* wavefront-uniform (scalar) values: tile ownership tests compile to
+   * scalar branches */
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wr = wave % WR, wc = wave / WR;
+  const int MT = a.MT;
+
+  float *sl = reinterpret_cast<float *>(smem + a.pre_base);
+  int *tb = reinterpret_cast<int *>(smem + a.pre_base + a.slot_bytes);
+  int *flagA = reinterpret_cast<int *>(smem + a.flag_base);
+  int *flagB = flagA + 1;
+  double *redd = reinterpret_cast<double *>(smem + a.red_base);
+  float *Xb = reinterpret_cast<float *>(smem + a.x_base);
+  float *Lb = reinterpret_cast<float *>(smem + a.l_base);
+  float *Ub = reinterpret_cast<float *>(smem + a.u_base);
+  float *LUb = reinterpret_cast<float *>(smem + a.lu_base);
+  float *Linvb = reinterpret_cast<float *>(smem + a.linv_base);
+  float *UinvTb = reinterpret_cast<float *>(smem + a.uinvt_base);
+  float *Urevb = reinterpret_cast<float *>(smem + a.urev_base);
+  float *pivb = reinterpret_cast<float *>(smem + a.piv_base); /* |u_kk| of the dense pivots */
+  float *stage = reinterpret_cast<float *>(smem + a.stage_base);
+  /* per-lane bases: a tile of wavefront-local index (ia, jb) sits at a
+   * compile-time offset from them (the LDS instructions' immediate), so
+   * the unrolled tile loops hold a handful of address registers */
+  const int frag = l16 * TR + 4 * g;           /* MFMA operand fragment   */
+  float *Lw = Lb + wr * TSF + frag;            /* tile I = wr + WR ia     */
+  float *Uw = Ub + wc * TSF + frag;            /* tile J = wc + WC jb     */
+  float *Xw = Xb + wr * TSF + 4 * g * TR + l16; /* accumulator layout     */
+  constexpr int SST = 16 * WC * CT + 4;        /* staging row stride      */
+
+  for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
+    const double *th = a.theta + (size_t)d * a.P;
+    int bad = 0;
+    float amax = 0.f;
+    double ls = 0.0;
+    if (tid == 0) {
+      *flagA = 0;
+      *flagB = 0;
+    }
+    prefix_eliminate<float>(sl, tb, th, a.pre, tid, NT, !a.tab_persist || d == (long)blockIdx.x, bad,
+                            amax, ls, flagA);
+    bool stop = *flagA != 0;
+
+    f32x4 acc[RT][CT];
+#pragma unroll
+    for (int ia = 0; ia < RT; ++ia)
+#pragma unroll
+      for (int jb = 0; jb < CT; ++jb)
+        acc[ia][jb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    /* ---- Schur complement into the accumulators, one LDS band at a time */
+    for (int band = 0; band < a.n_band && !stop; ++band) {
+      const int r0 = band * a.band_rows;
+      const int rows = min(a.band_rows, 16 * MT - r0);
+      const int nvec = (rows * SST + 3) >> 2;
+      f32x4 *s4 = reinterpret_cast<f32x4 *>(stage);
+      for (int q = tid; q < nvec; q += NT)
+        s4[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+      __syncthreads();
+      for (int r = tid; r < rows; r += NT)
+        stage[r * SST + r0 + r] = 1.f;
+      for (int e = a.sg_ptr[band] + tid; e < a.sg_ptr[band + 1]; e += NT)
+        stage[a.sg_off[e]] = (float)(-th[a.sg_idx[e]]);
+      __syncthreads();
+      const int b1 = a.blk_ptr[band + 1];
+      for (int b0 = a.blk_ptr[band] + wave; b0 < b1; b0 += NW * UB) {
+        int off[UB], R[UB], cb[UB];
+        unsigned pv[UB][RR];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          const int b = b0 + NW * u;
+          const bool ok = b < b1;
+          R[u] = ok ? a.brounds[b] : 0;
+          cb[u] = ok ? a.bcbase[b] : 0;
+          off[u] = ok ? a.st_off[(size_t)b * 64 + lane] : -1;
+#pragma unroll
+          for (int r = 0; r < RR; ++r)
+            pv[u][r] = r < R[u] ? a.st_pairs[(size_t)cb[u] + 64 * r + lane] : a.zero_pair;
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+          float s = 0.f;
+#pragma unroll
+          for (int r = 0; r < RR; ++r)
+            s = fmaf(-sl[pv[u][r] & LO], sl[pv[u][r] >> 16], s);
+          for (int rb = RR; rb < R[u]; rb += RR) {
+            unsigned c[RR];
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+              c[r] = rb + r < R[u] ? a.st_pairs[(size_t)cb[u] + 64 * (rb + r) + lane] : a.zero_pair;
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+              s = fmaf(-sl[c[r] & LO], sl[c[r] >> 16], s);
+          }
+          if (off[u] >= 0)
+            stage[off[u]] = s;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ia = 0; ia < RT; ++ia) {
+        const int I = wr + WR * ia;
+        if (I < MT && 16 * I >= r0 && 16 * I < r0 + rows) {
+          const float *srow = stage + (16 * I - r0 + 4 * g) * SST + 16 * wc + l16;
+#pragma unroll
+          for (int jb = 0; jb < CT; ++jb) {
+            const int J = wc + WC * jb;
+            if (J < MT) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                acc[ia][jb][r] = srow[r * SST + 16 * WC * jb];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    /* ---- phase 1 of step k: the column-k tiles to LDS (X), and the
+     * diagonal tile's factors from its owner */
+    auto phase1 = [&](int k) {
+      if (wc == k % WC) {
+        const int jk = k / WC;
+#pragma unroll
+        for (int jb = 0; jb < CT; ++jb) {
+          if (jb == jk) {
+#pragma unroll
+            for (int ia = 0; ia < RT; ++ia) {
+              const int I = wr + WR * ia;
+              if (I >= k && I < MT) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  Xw[WR * ia * TSF + r * TR] = acc[ia][jb][r];
+              }
+            }
+          }
+        }
+        if (wr == k % WR) {
+          wave_lds_sync();
+          factor_diag_tile((const lds_f *)(Xb + k * TSF), (lds_f *)LUb, (lds_f *)Urevb,
+                           (lds_f *)Linvb, (lds_f *)UinvTb, (lds_f *)(pivb + 16 * k), lane, bad);
+          if (bad)
+            *flagA = 1;
+        }
+      }
+    };
+
+    /* A_IJ -= L_Ik U_kJ for the owned tiles of the columns J that sel(J)
+     * picks, rows I > k */
+    auto update = [&](int k, auto sel) {
+#pragma unroll
+      for (int jb = 0; jb < CT; ++jb) {
+        const int J = wc + WC * jb;
+        if (J < MT && sel(J)) {
+          if constexpr (MODE == RS_FP32) {
+            const f32x4 ub = *reinterpret_cast<const f32x4 *>(Uw + WC * jb * TSF);
+#pragma unroll
+            for (int ia = 0; ia < RT; ++ia) {
+              const int I = wr + WR * ia;
+              if (I > k && I < MT) {
+                const f32x4 la = *reinterpret_cast<const f32x4 *>(Lw + WR * ia * TSF);
+                f32x4 c = acc[ia][jb];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  c = mfma_f32(la[r], ub[r], c);
+                acc[ia][jb] = c;
+              }
+            }
+          } else {
+            const u16x8 ub = *reinterpret_cast<const u16x8 *>(Uw + WC * jb * TSF);
+            const s16x4 uh = hi_part(ub), ul = lo_part(ub);
+#pragma unroll
+            for (int ia = 0; ia < RT; ++ia) {
+              const int I = wr + WR * ia;
+              if (I > k && I < MT) {
+                const u16x8 la = *reinterpret_cast<const u16x8 *>(Lw + WR * ia * TSF);
+                const s16x4 lh = hi_part(la), ll = lo_part(la);
+                f32x4 c = acc[ia][jb];
+                c = mfma_bf16(ll, uh, c);
+                c = mfma_bf16(lh, ul, c);
+                c = mfma_bf16(lh, uh, c);
+                acc[ia][jb] = c;
+              }
+            }
+          }
+        }
+      }
+    };
+
+    if (!stop)
+      phase1(0);
+    for (int k = 0; k < MT && !stop; ++k) {
+      __syncthreads();
+      if (*flagA) {
+        stop = true;
+        break;
+      }
+      /* ---- phase 3: U_kJ by the owners of row k, L_Ik by the others */
+      if (wr == k % WR) {
+        const f32x4 li = *reinterpret_cast<const f32x4 *>(Linvb + frag);
+        const int ik = k / WR;
+#pragma unroll
+        for (int ia = 0; ia < RT; ++ia) {
+          if (ia == ik) {
+#pragma unroll
+            for (int jb = 0; jb < CT; ++jb) {
+              const int J = wc + WC * jb;
+              if (J > k && J < MT) {
+                f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                  c = mfma_f32(li[r], acc[ia][jb][r], c);
+                /* U_kJ transposed: row l16 holds U[4g .. 4g + 3][l16] */
+                if constexpr (MODE == RS_FP32) {
+                  *reinterpret_cast<f32x4 *>(Uw + WC * jb * TSF) = c;
+                } else {
+                  u16x8 h;
+#pragma unroll
+                  for (int r = 0; r < 4; ++r) {
+                    const unsigned short hi = to_bf16(c[r]);
+                    h[r] = hi;
+                    h[4 + r] = to_bf16(c[r] - from_bf16(hi));
+                  }
+                  *reinterpret_cast<u16x8 *>(Uw + WC * jb * TSF) = h;
+                }
+              }
+            }
+          }
+        }
+      } else {
+        const f32x4 ui = *reinterpret_cast<const f32x4 *>(UinvTb + frag);
+        /* the (WR - 1) WC wavefronts outside row k take the column tiles
+         * in turn */
+        const int rank = ((wr - k % WR + WR) % WR - 1) * WC + wc;
+        for (int t = rank; k + 1 + t < MT; t += (WR - 1) * WC) {
+          const int I = k + 1 + t;
+          const f32x4 xa = *reinterpret_cast<const f32x4 *>(Xb + I * TSF + frag);
+          f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            c = mfma_f32(xa[r], ui[r], c);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (!(fabsf(c[r]) <= kMaxMultiplier))
+              bad = 1;
+          /* -L_Ik row-major: row 4g + r, column l16 */
+          if constexpr (MODE == RS_FP32) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              Lb[I * TSF + (4 * g + r) * TR + l16] = -c[r];
+          } else {
+            unsigned short *row = reinterpret_cast<unsigned short *>(Lb + I * TSF);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float x = -c[r];
+              const unsigned short hi = to_bf16(x);
+              /* element (4g + r, l16): k-group l16 / 4, position l16 % 4 */
+              const int o = (4 * g + r) * (2 * TR) + (l16 >> 2) * 8 + (l16 & 3);
+              row[o] = hi;
+              row[o + 4] = to_bf16(x - from_bf16(hi));
+            }
+          }
+        }
+      }
+      if (bad)
+        *flagB = 1;
+      __syncthreads();
+      if (*flagB) {
+        stop = true;
+        break;
+      }
+      /* ---- phase 4: column k + 1 first, its diagonal tile factored while
+       * the remaining columns are updated */
+      if (k + 1 < MT) {
+        update(k, [&](int J) { return J == k + 1; });
+        phase1(k + 1);
+        update(k, [&](int J) { return J > k + 1; });
+      }
+    }
+
+    /* ---- log|det| (FP64, fixed reduction order) and the fallback list */
+    __syncthreads();
+    if (!stop)
+      for (int q = tid; q < 16 * MT; q += NT)
+        ls += log((double)pivb[q]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+      ls += __shfl_xor(ls, o);
+    if (lane == 0)
+      redd[wave] = ls;
+    __syncthreads();
+    if (tid == 0) {
+      double logdet = redd[0];
+      for (int w = 1; w < NW; ++w)
+        logdet += redd[w];
+      if (stop) {
+        const int slot = atomicAdd(a.flag_count, 1);
+        a.flag_list[slot] = (int)d;
+      }
+      a.logdet[d] = stop ? NAN : logdet;
+    }
+    __syncthreads();
+  }
+}
+
+} // namespace
+
+size_t rs_panel_bytes(int MT) { return (size_t)(3 * MT + 4) * TSF * sizeof(float) + 64 * MT; }
+
+/* The instantiation that serves MT tiles per dimension: wavefront grid
+ * WR x WC (eight wavefronts, two per SIMD, 256 registers each) and
+ * accumulator tiles RT x CT per wavefront. */
+struct RsShape {
+  int WR, WC, RT, CT;
+};
+static RsShape rs_shape(int MT) {
+  if (MT <= 12)
+    return {2, 4, 6, 3};
+  if (MT <= 16)
+    return {2, 4, 8, 4};
+  return {2, 4, 9, 5};
+}
+
+int rs_stage_stride(int MT) {
+  const RsShape q = rs_shape(MT);
+  return 16 * q.WC * q.CT + 4;
+}
+
+int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                 double *logdet, int *flags, hipStream_t st) {
+  RsArgs a{};
+  a.MT = plan.rs_MT;
+  a.n_draws = N;
+  a.theta = theta;
+  a.P = S.P;
+  a.logdet = logdet;
+  a.flag_count = flags;
+  a.flag_list = flags + 1;
+  a.pre = PrefixTabs{S.e_ns, S.e_waves, S.e_n1, S.e_tab_words, S.e_o_piv, S.e_o_mptr, S.e_o_mult,
+                     S.e_o_tptr, S.e_o_tslot, S.e_o_cptr, S.e_o_contrib, S.e_slot_init, S.e_tab};
+  a.pre_base = 0;
+  a.slot_bytes = (int)((((size_t)S.e_ns * 4) + 15) & ~(size_t)15);
+  a.tab_persist = 1;
+  a.flag_base = (int)plan.rs_flag_base;
+  a.red_base = a.flag_base + 16;
+  const int P0 = (int)plan.rs_panel_base;
+  a.x_base = P0;
+  a.l_base = a.x_base + a.MT * TSF * 4;
+  a.u_base = a.l_base + a.MT * TSF * 4;
+  a.lu_base = a.u_base + a.MT * TSF * 4;
+  a.linv_base = a.lu_base + TSF * 4;
+  a.uinvt_base = a.linv_base + TSF * 4;
+  a.urev_base = a.uinvt_base + TSF * 4;
+  a.piv_base = a.urev_base + TSF * 4;
+  a.stage_base = P0;
+  a.band_rows = S.e_rb_rows;
+  a.n_band = S.e_rb_nband;
+  a.blk_ptr = S.e_rb_blk_ptr;
+  a.st_off = S.e_rb_st_off;
+  a.brounds = S.e_rb_brounds;
+  a.bcbase = S.e_rb_bcbase;
+  a.sg_ptr = S.e_rb_sg_ptr;
+  a.sg_off = S.e_rb_sg_off;
+  a.sg_idx = S.e_rb_sg_idx;
+  a.st_pairs = S.e_rb_pairs;
+  a.zero_pair = S.e_zero_pair;
+  void (*kern)(RsArgs) = nullptr;
+  const bool bf = plan.prec == SGDLM_PREC_BF16X3;
+  const RsShape q = rs_shape(a.MT);
+  if (a.MT > kRsMaxTiles)
+    return SGDLM_ERR_ARG;
+  if (q.RT == 6)
+    kern = bf ? k_lu_rs<2, 4, 6, 3, RS_BF16X3> : k_lu_rs<2, 4, 6, 3, RS_FP32>;
+  else if (q.RT == 8)
+    kern = bf ? k_lu_rs<2, 4, 8, 4, RS_BF16X3> : k_lu_rs<2, 4, 8, 4, RS_FP32>;
+  else
+    kern = bf ? k_lu_rs<2, 4, 9, 5, RS_BF16X3> : k_lu_rs<2, 4, 9, 5, RS_FP32>;
+  const int threads = 64 * q.WR * q.WC;
+  if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)plan.rs_lds) != hipSuccess)
+    return SGDLM_ERR_DEVICE;
+  hipLaunchKernelGGL(kern, dim3(plan.rs_grid), dim3(threads), plan.rs_lds, st, a);
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
+} // namespace sgdlm

@@ -388,7 +388,7 @@ void launch_weights(long N, const double *logdet, double *w, double *stats,
 
 void launch_moments(const Structure &S, long N, const double *lam,
                     const double *theta, const double *m_t, const double *n_t,
-                    const double *s_t, VbBuffers &B, hipStream_t st) {
+                    const double *s_t, const VbBuffers &B, hipStream_t st) {
   const int chunks = B.chunks;
   const int nblk = (S.max_p + 1 + 15) / 16;
   const dim3 grid(chunks, S.m);

@@ -288,4 +288,65 @@ void build_elim_plan(int m, const std::vector<GammaEntry> &ents, int max_front,
   }
 }
 
+void build_band_plan(const ElimPlan &P, int BR, int stride, BandPlan &B) {
+  B = BandPlan();
+  B.BR = BR;
+  B.stride = stride;
+  B.n_band = P.M2 > 0 ? (P.M2 + BR - 1) / BR : 0;
+  const int *brounds = P.tab.data() + P.o_brounds, *bcbase = P.tab.data() + P.o_bcbase;
+  /* per band: (pair list, band-local offset) of every Schur target */
+  std::vector<std::vector<std::pair<std::vector<uint32_t>, int>>> tg(B.n_band);
+  for (int b = 0; b < P.n_blk; ++b)
+    for (int l = 0; l < 64; ++l) {
+      const int o = P.st_off[(size_t)b * 64 + l];
+      if (o < 0)
+        continue;
+      const int r = o / P.M2, c = o % P.M2;
+      std::vector<uint32_t> v;
+      for (int q = 0; q < brounds[b]; ++q) {
+        const uint32_t x = P.st_pairs[(size_t)bcbase[b] + (size_t)q * 64 + l];
+        if (q == 0 || x != P.zero_pair)
+          v.push_back(x);
+      }
+      tg[r / BR].push_back({std::move(v), (r % BR) * stride + c});
+    }
+  B.blk_ptr.assign(1, 0);
+  for (int band = 0; band < B.n_band; ++band) {
+    auto &t = tg[band];
+    std::stable_sort(t.begin(), t.end(), [](const auto &x, const auto &y) {
+      return x.first.size() > y.first.size();
+    });
+    for (size_t t0 = 0; t0 < t.size(); t0 += 64) {
+      const int R0 = (int)t[t0].first.size();
+      B.brounds.push_back(R0);
+      B.bcbase.push_back((int)B.st_pairs.size());
+      B.st_pairs.resize(B.st_pairs.size() + (size_t)R0 * 64, P.zero_pair);
+      const size_t b0 = B.st_off.size();
+      B.st_off.resize(b0 + 64, -1);
+      for (int l = 0; l < 64 && t0 + l < t.size(); ++l) {
+        const auto &e = t[t0 + l];
+        B.st_off[b0 + l] = e.second;
+        for (size_t q = 0; q < e.first.size(); ++q)
+          B.st_pairs[(size_t)B.bcbase.back() + q * 64 + l] = e.first[q];
+      }
+    }
+    B.blk_ptr.push_back((int)B.brounds.size());
+  }
+  B.n_blk = (int)B.brounds.size();
+  /* Gamma entries of S by band */
+  std::vector<std::vector<std::pair<int, int>>> sg(B.n_band);
+  for (size_t e = 0; e < P.sg_off.size(); ++e) {
+    const int r = P.sg_off[e] / P.M2, c = P.sg_off[e] % P.M2;
+    sg[r / BR].push_back({(r % BR) * stride + c, P.sg_idx[e]});
+  }
+  B.sg_ptr.assign(1, 0);
+  for (int band = 0; band < B.n_band; ++band) {
+    for (const auto &e : sg[band]) {
+      B.sg_off.push_back(e.first);
+      B.sg_idx.push_back(e.second);
+    }
+    B.sg_ptr.push_back((int)B.sg_off.size());
+  }
+}
+
 } // namespace sgdlm

@@ -94,6 +94,26 @@ struct ElimPlan {
   std::vector<uint32_t> bwd;  /* slot(u) | x index << 16                   */
 };
 
+/* The Schur-complement tables of a plan regrouped for the
+ * register-resident LU, which stages S through LDS one band of BR rows at a
+ * time (row stride `stride` floats): the targets and the Gamma entries of
+ * each band, with offsets local to the band.  Blocks of 64 targets (ELL,
+ * round r of block b at bcbase[b] + 64 r) are sorted by rounds within a
+ * band; a band's blocks are blk_ptr[band] .. blk_ptr[band + 1]. */
+struct BandPlan {
+  int BR = 0, stride = 0, n_band = 0, n_blk = 0;
+  std::vector<int> blk_ptr;       /* n_band + 1                             */
+  std::vector<int> st_off;        /* n_blk * 64 band-local offsets, -1 pad  */
+  std::vector<uint32_t> st_pairs; /* pairs (x | y << 16), zero_pair padding */
+  std::vector<int> brounds, bcbase; /* n_blk                                */
+  std::vector<int> sg_ptr;        /* n_band + 1                             */
+  std::vector<int> sg_off, sg_idx; /* Gamma entries of S, band-local        */
+};
+
+/* Regroups P's Schur tables into bands of BR rows (BR a multiple of 16)
+ * with LDS row stride `stride` (>= P.M2). */
+void build_band_plan(const ElimPlan &P, int BR, int stride, BandPlan &B);
+
 /* Builds the plan.  max_front = largest degree a prefix pivot may have
  * (0 disables the prefix: n1 = 0, S = the whole matrix); max_slots bounds
  * the LDS slots (and must be <= 65536). */

@@ -149,7 +149,8 @@ struct StageTimer {
   std::string stage;
   hipEvent_t a{}, b{};
   bool on;
-  StageTimer(sgdlm_ctx *ctx, const char *s) : c(ctx), stage(s), on(ctx->timing) {
+  StageTimer(sgdlm_ctx *ctx, const char *s, bool enabled = true)
+      : c(ctx), stage(s), on(enabled && ctx->timing) {
     if (on) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
@@ -296,6 +297,55 @@ template <typename V> int upload(sgdlm_ctx *c, const char *name, const V &v) {
   return SGDLM_OK;
 }
 
+/* LDS layout and banded Schur tables of the register-resident LU
+ * (k_lu_rs.hip) for plan P: the prefix slots and tables, two flags and the
+ * reduction words, then one region shared by the staged panel tiles and
+ * the staging band of S (as many 16-row groups as fit).  Leaves
+ * e_rb_rows = 0 when the dense block is outside 6..18 tiles or does not
+ * fit. */
+int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
+  Structure &S = c->S;
+  S.e_rb_rows = 0;
+  const int MT = P.M2 / 16;
+  if (MT < 6 || MT > kRsMaxTiles)
+    return SGDLM_OK;
+  const size_t pre = (((size_t)P.ns * 4 + 15) & ~(size_t)15) + P.tab.size() * 4;
+  const size_t flag_base = (pre + 15) & ~(size_t)15;
+  const size_t panel_base = flag_base + 16 + 8 * sizeof(double);
+  const size_t panel = rs_panel_bytes(MT);
+  const int stride = rs_stage_stride(MT);
+  int BR = 0;
+  for (int br = 16; br <= 16 * MT; br += 16)
+    if (panel_base + std::max(panel, (size_t)br * stride * 4) <= c->lds_max)
+      BR = br;
+  if (BR == 0)
+    return SGDLM_OK;
+  BandPlan B;
+  build_band_plan(P, BR, stride, B);
+  int rc;
+  if ((rc = upload(c, "rb_blk_ptr", B.blk_ptr)) || (rc = upload(c, "rb_st_off", B.st_off)) ||
+      (rc = upload(c, "rb_pairs", B.st_pairs)) || (rc = upload(c, "rb_brounds", B.brounds)) ||
+      (rc = upload(c, "rb_bcbase", B.bcbase)) || (rc = upload(c, "rb_sg_ptr", B.sg_ptr)) ||
+      (rc = upload(c, "rb_sg_off", B.sg_off)) || (rc = upload(c, "rb_sg_idx", B.sg_idx)))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  S.e_rb_blk_ptr = M_<int>(c, "rb_blk_ptr");
+  S.e_rb_st_off = M_<int>(c, "rb_st_off");
+  S.e_rb_pairs = M_<uint32_t>(c, "rb_pairs");
+  S.e_rb_brounds = M_<int>(c, "rb_brounds");
+  S.e_rb_bcbase = M_<int>(c, "rb_bcbase");
+  S.e_rb_sg_ptr = M_<int>(c, "rb_sg_ptr");
+  S.e_rb_sg_off = M_<int>(c, "rb_sg_off");
+  S.e_rb_sg_idx = M_<int>(c, "rb_sg_idx");
+  S.e_rb_stride = stride;
+  S.e_rb_nband = B.n_band;
+  S.e_rb_flag_base = flag_base;
+  S.e_rb_panel_base = panel_base;
+  S.e_rb_lds = panel_base + std::max(panel, ((size_t)BR * stride * 4 + 15) & ~(size_t)15);
+  S.e_rb_rows = BR;
+  return SGDLM_OK;
+}
+
 /* Symbolic elimination plan of the VB log-determinant LU for the current
  * parental sets (lu_symbolic.h).  Its LDS slots (FP64) and tables must fit
  * the LDS of one workgroup; the front limit is halved until they do. */
@@ -372,7 +422,7 @@ int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
   S.e_fwd = M_<uint32_t>(c, "e_fwd");
   S.e_bwd_ptr = M_<int>(c, "e_bwd_ptr");
   S.e_bwd = M_<uint32_t>(c, "e_bwd");
-  return SGDLM_OK;
+  return build_band_tables(c, P);
 }
 
 /* Rebuild the device-side parent structure from h_p / h_sp. */
@@ -442,49 +492,70 @@ int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
   return dalloc(c, c->sim, "lu_work", plan->work_bytes);
 }
 
-/* The VB stages on the device (no host copies). */
+/* Device buffers of one VB call beyond the model state (the context's
+ * simulation pool in run_vb). */
+struct VbDev {
+  double *lam = nullptr, *theta = nullptr, *logdet = nullptr;
+  int *lu_flags = nullptr;
+  void *lu_work = nullptr;
+  VbBuffers B;
+};
+
+/* The VB stages on the device (no host copies) for N draws; ends with the
+ * weight statistics copied to the pinned c->h_stats. */
+int run_vb_stages(sgdlm_ctx *c, long N, const VbDev &D, const LuPlan &plan, bool timed) {
+  int rc;
+  if ((rc = check_epochs(c, 1)))
+    return rc;
+  const uint32_t ep = ++c->epoch;
+  const bool t_on = timed && c->timing;
+  {
+    StageTimer t(c, "draw", t_on);
+    launch_cholesky(c->S, M_<double>(c, "C"), M_<double>(c, "U"), c->stream);
+    launch_draw(c->S, N, M_<double>(c, "m"), M_<double>(c, "U"), M_<double>(c, "n"),
+                M_<double>(c, "s"), c->seed, ep, D.lam, D.theta, c->stream);
+  }
+  CHECK_LAUNCH(c);
+  {
+    StageTimer t(c, "lu", t_on);
+    rc = launch_lu_logdet(c->S, plan, N, D.theta, D.lu_work, D.logdet, D.lu_flags, c->stream);
+  }
+  if (rc)
+    return fail(c, rc, "LU launch failed: %s", hipGetErrorString(hipGetLastError()));
+  {
+    StageTimer t(c, "weights", t_on);
+    launch_weights(N, D.logdet, D.B.w, D.B.stats, c->stream);
+  }
+  CHECK_LAUNCH(c);
+  {
+    StageTimer t(c, "moments", t_on);
+    launch_moments(c->S, N, D.lam, D.theta, M_<double>(c, "m"), M_<double>(c, "n"),
+                   M_<double>(c, "s"), D.B, c->stream);
+  }
+  CHECK_LAUNCH(c);
+  HIPCHK(c, hipMemcpyAsync(c->h_stats, D.B.stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  return SGDLM_OK;
+}
+
+/* VB on the context's simulation pool (results stay on the device). */
 int run_vb(sgdlm_ctx *c) {
   int rc;
   if ((rc = ensure_sim_buffers(c)))
     return rc;
-  const long N = (long)c->nsim;
-  if ((rc = check_epochs(c, 1)))
-    return rc;
-  const uint32_t ep = ++c->epoch;
-  {
-    StageTimer t(c, "draw");
-    launch_cholesky(c->S, M_<double>(c, "C"), M_<double>(c, "U"), c->stream);
-    launch_draw(c->S, N, M_<double>(c, "m"), M_<double>(c, "U"),
-                M_<double>(c, "n"), M_<double>(c, "s"), c->seed, ep,
-                SIM<double>(c, "lam"), SIM<double>(c, "theta"), c->stream);
-  }
-  CHECK_LAUNCH(c);
   LuPlan plan;
   if ((rc = lu_workspace(c, c->prec, 0, &plan)))
     return rc;
-  {
-    StageTimer t(c, "lu");
-    rc = launch_lu_logdet(c->S, plan, N, SIM<double>(c, "theta"),
-                          c->sim["lu_work"].ptr, SIM<double>(c, "logdet"),
-                          SIM<int>(c, "lu_flags"), c->stream);
-  }
-  if (rc)
-    return fail(c, rc, "LU launch failed: %s", hipGetErrorString(hipGetLastError()));
+  VbDev D;
+  D.lam = SIM<double>(c, "lam");
+  D.theta = SIM<double>(c, "theta");
+  D.logdet = SIM<double>(c, "logdet");
+  D.lu_flags = SIM<int>(c, "lu_flags");
+  D.lu_work = c->sim["lu_work"].ptr;
+  D.B = vb_buffers(c);
+  if ((rc = run_vb_stages(c, (long)c->nsim, D, plan, true)))
+    return rc;
   c->lu_flags_valid = true;
-  VbBuffers B = vb_buffers(c);
-  {
-    StageTimer t(c, "weights");
-    launch_weights(N, SIM<double>(c, "logdet"), B.w, B.stats, c->stream);
-  }
-  CHECK_LAUNCH(c);
-  {
-    StageTimer t(c, "moments");
-    launch_moments(c->S, N, SIM<double>(c, "lam"), SIM<double>(c, "theta"),
-                   M_<double>(c, "m"), M_<double>(c, "n"), M_<double>(c, "s"), B, c->stream);
-  }
-  CHECK_LAUNCH(c);
-  HIPCHK(c, hipMemcpyAsync(c->h_stats, B.stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
   c->vb_valid = true;
   return SGDLM_OK;
 }

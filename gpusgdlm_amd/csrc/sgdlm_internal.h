@@ -39,6 +39,15 @@ struct Structure {
   const uint32_t *e_st_pairs = nullptr;
   const int *e_wave_q = nullptr, *e_node_of = nullptr, *e_fwd_ptr = nullptr, *e_bwd_ptr = nullptr;
   const uint32_t *e_fwd = nullptr, *e_bwd = nullptr;
+  /* The Schur tables regrouped by LDS staging band for the
+   * register-resident LU (lu_symbolic.h BandPlan); e_rb_rows = 0: that
+   * kernel does not fit this structure. */
+  int e_rb_rows = 0, e_rb_stride = 0, e_rb_nband = 0;
+  size_t e_rb_lds = 0, e_rb_panel_base = 0, e_rb_flag_base = 0;
+  const int *e_rb_blk_ptr = nullptr, *e_rb_st_off = nullptr, *e_rb_brounds = nullptr,
+            *e_rb_bcbase = nullptr, *e_rb_sg_ptr = nullptr, *e_rb_sg_off = nullptr,
+            *e_rb_sg_idx = nullptr;
+  const uint32_t *e_rb_pairs = nullptr;
 };
 
 /* ---- filter (k_filter.hip) ---- */
@@ -91,6 +100,10 @@ struct LuPlan {
   size_t slot_stride = 0;     /* bytes per workgroup                        */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
   size_t work_bytes = 0;      /* global workspace bytes                      */
+  /* register-resident LU of the dense block (k_lu_rs.hip) instead of the
+   * fast kernel: rs_MT > 0 */
+  int rs_MT = 0, rs_grid = 0;
+  size_t rs_lds = 0, rs_panel_base = 0, rs_flag_base = 0;
 };
 /* want_threads / want_nb: 0 = automatic, else the fast kernel's workgroup
  * size (256 or 512) and panel width; an infeasible request fails. */
@@ -99,6 +112,17 @@ struct LuPlan {
 int lu_dense_cap(size_t lds_max);
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             int want_threads, int want_nb, LuPlan *plan);
+/* LDS layout of the register-resident LU for a dense block of MT 16 x 16
+ * tiles: staged panel tiles (bytes, k_lu_rs.hip), and the largest number
+ * of tiles per dimension it handles. */
+size_t rs_panel_bytes(int MT);
+/* row stride (floats) of the register-resident LU's LDS staging band */
+int rs_stage_stride(int MT);
+constexpr int kRsMaxTiles = 18;
+/* Register-resident LU log-determinants (plan.rs_MT > 0); flagged draws are
+ * appended to flags for the partial-pivoting kernel. */
+int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                 double *logdet, int *flags, hipStream_t st);
 /* flags: N + 1 ints of scratch (count + list of draws handed to the
  * partial-pivoting kernel). */
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
@@ -129,7 +153,7 @@ void launch_weights(long N, const double *logdet, double *w, double *stats,
                     hipStream_t st);
 void launch_moments(const Structure &S, long N, const double *lam,
                     const double *theta, const double *m_t, const double *n_t,
-                    const double *s_t, VbBuffers &B, hipStream_t st);
+                    const double *s_t, const VbBuffers &B, hipStream_t st);
 
 } // namespace sgdlm
 

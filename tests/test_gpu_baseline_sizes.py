@@ -223,10 +223,11 @@ def test_readme_loop_T500(hiplib):
     1e4-draw forecast with its 5/95% quantiles, at every step, GPU and oracle
     each carrying their own state.  Bars: VB m/C/n/s 1e-8 relative after up
     to 499 coupled steps; the accept decisions identical; forecast quantiles
-    1e-8; per draw, relative error <= 1e-10 or <= 1e-12 kappa (the two
-    states differ by ~1e-11 after the coupled steps and the solve amplifies
-    that by the condition number kappa of I - Gamma; the README model has
-    2-cycles, so near-singular draws occur)."""
+    1e-8; per draw, relative error <= 1e-10 max(1, kappa): the two states
+    drift apart by up to ~1e-10 in single entries over the coupled steps
+    (reported as the parameter drift) and the solve amplifies a perturbation
+    of the draw by the condition number kappa of its I - Gamma; the README
+    model has 2-cycles, so ill-conditioned draws occur."""
     mod = readme_model(T=500)
     T, N = 500, 10000
     g = SGDLM(strict=True)
@@ -235,7 +236,7 @@ def test_readme_loop_T500(hiplib):
     for api in (g, o):
         configure(api, mod)
     accepted = 0
-    worst = {"vb": 0.0, "q": 0.0, "y_easy": 0.0, "kappa_max": 0.0}
+    worst = {"vb": 0.0, "q": 0.0, "y_easy": 0.0, "kappa_max": 0.0, "drift": 0.0}
     n_hard = 0
     for t in range(1, T):
         Ft = mod.F(t)
@@ -254,6 +255,10 @@ def test_readme_loop_T500(hiplib):
             o.setPosteriorParameters(vo["m"], vo["C"], vo["n"], vo["s"])
         for api in (g, o):
             api.computePrior()
+        pg, po = g.getParameters(), o.getParameters()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            d = max(float(np.nanmax(np.abs(pg[k] - po[k]) / np.abs(po[k]))) for k in ("m", "C", "n", "s"))
+        worst["drift"] = max(worst["drift"], d)
         if t < T - 1:
             Fn = mod.F_forecast(t + 1)
             rg = g.computeForecastDebug(N, N, Fn)
@@ -267,7 +272,7 @@ def test_readme_loop_T500(hiplib):
                 n_hard += len(hard)
                 kap = _kappas(mod, rg["thetas"][:, :, hard])
                 worst["kappa_max"] = max(worst["kappa_max"], float(kap.max()))
-                assert np.all(err[hard] <= 1e-12 * kap), (t, err[hard].max(), kap.min())
+                assert np.all(err[hard] <= 1e-10 * kap), (t, err[hard].max(), kap.min())
             qg = np.quantile(yg, [0.05, 0.95], axis=1)
             qo = np.quantile(yo, [0.05, 0.95], axis=1)
             worst["q"] = max(worst["q"], relerr(qg, qo))
