@@ -58,6 +58,8 @@ SIGNATURES = {
     "sgdlm_compute_forecast": (C.c_int, [_ctx, _dp, _dp, _dp, _dp, C.c_int]),
     "sgdlm_compute_posterior": (C.c_int, [_ctx, _dp, _dp]),
     "sgdlm_compute_vb_posterior": (C.c_int, [_ctx, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "sgdlm_compute_vb_posterior_noalloc": (C.c_int, [_ctx, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
+                                                     C.POINTER(C.c_size_t)]),
     "sgdlm_set_precision": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_get_precision": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
     "sgdlm_set_rng_state": (C.c_int, [_ctx, C.c_uint64, C.c_uint32]),

@@ -1579,6 +1579,29 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   return SGDLM_OK;
 }
 
+int lu_plan_cap(const Structure &S, size_t work_cap, LuPlan *plan) {
+  const size_t ts = (plan->prec == SGDLM_PREC_FP64) ? 8 : 4;
+  const size_t wg_full = (size_t)S.M * S.M * ts;   /* partial-pivoting kernel */
+  const size_t wg_fast = (size_t)plan->fast_M * plan->fast_M * ts + plan->slot_stride;
+  const size_t wg = std::max(wg_full, plan->nb ? wg_fast : (size_t)0) + 256;
+  if (plan->work_bytes <= work_cap)
+    return SGDLM_OK;
+  const int max_wg = (int)std::min<size_t>(work_cap / wg, 1 << 20);
+  if (max_wg < 1)
+    return SGDLM_ERR_ARG;
+  plan->grid = std::min(plan->grid, max_wg);
+  plan->pivot_grid = std::min(plan->pivot_grid, max_wg);
+  if (plan->nb)
+    plan->fast_grid = std::min(plan->fast_grid, max_wg);
+  plan->work_bytes =
+      (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * S.M * S.M * ts;
+  if (plan->slot_stride) {
+    plan->slot_save_off = (plan->work_bytes + 255) & ~(size_t)255;
+    plan->work_bytes = plan->slot_save_off + (size_t)plan->fast_grid * plan->slot_stride;
+  }
+  return plan->work_bytes <= work_cap ? SGDLM_OK : SGDLM_ERR_ARG;
+}
+
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
                      const double *theta, void *work, double *logdet,
                      int *flags, hipStream_t st) {

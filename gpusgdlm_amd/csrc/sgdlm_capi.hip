@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,7 +31,40 @@
 
 using namespace sgdlm;
 
+/* Device memory of sgdlm_compute_vb_posterior_noalloc: every buffer of a
+ * VB call beyond the model state and the caller's outputs is carved from
+ * this array instead of being allocated.  Each GPU the library runs on
+ * holds its own copy until the process ends.  In host code the name is
+ * only a symbol handle: kernels get the per-device address that
+ * hipGetSymbolAddress returns (scratch_ptr below). */
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr; /* device -> address of scratch */
+
+/* Address of this device's scratch array (with that device current);
+ * looked up once per device. */
+hipError_t scratch_ptr(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return e;
+  }
+  g_scratch_ptr[device] = p;
+  *out = p;
+  return hipSuccess;
+}
 
 struct DBuf {
   void *ptr = nullptr;
@@ -54,6 +88,8 @@ struct sgdlm_ctx {
   /* pinned host copy of the last VB call's weight statistics (VbBuffers
    * stats: max log-det, sum exp(l - max), sum w^2, zero weights) */
   double *h_stats = nullptr;
+  /* this device's scratch array (sgdlm_compute_vb_posterior_noalloc) */
+  void *scratch = nullptr;
 
   size_t m = 0, max_p = 0;
   bool memory_initialized = false;
@@ -492,8 +528,9 @@ int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
   return dalloc(c, c->sim, "lu_work", plan->work_bytes);
 }
 
-/* Device buffers of one VB call beyond the model state (the context's
- * simulation pool in run_vb). */
+/* Device buffers of one VB call beyond the model state: the context's
+ * simulation pool (run_vb) or the static scratch array (the allocation-free
+ * entry point). */
 struct VbDev {
   double *lam = nullptr, *theta = nullptr, *logdet = nullptr;
   int *lu_flags = nullptr;
@@ -608,6 +645,8 @@ int sgdlm_create(sgdlm_ctx **out, size_t no_gpus) {
     delete c;
     return SGDLM_ERR_DEVICE;
   }
+  if (scratch_ptr(c->device, &c->scratch) != hipSuccess)
+    c->scratch = nullptr; /* the allocation-free entry point then fails */
   if (hipHostMalloc((void **)&c->h_stats, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamDestroy(c->stream);
@@ -1028,6 +1067,82 @@ int sgdlm_compute_vb_posterior(sgdlm_ctx *c, double *mean_m, double *mean_C,
   }
   if ((rc = sync_and_collect(c)))
     return rc;
+  if (sum_det)
+    *sum_det = std::exp(c->h_stats[0]) * c->h_stats[1];
+  return vb_status(c);
+}
+
+int sgdlm_compute_vb_posterior_noalloc(sgdlm_ctx *c, size_t nsim, double *mean_m,
+                                       double *mean_C, double *mean_n, double *mean_s,
+                                       double *w, double *sum_det, size_t *scratch_bytes) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  int rc;
+  if ((rc = check_init(c)) || (rc = check_parents(c)) || (rc = check_prior(c, false)))
+    return rc;
+  if (nsim < 1 || nsim >= 0x80000000ull)
+    return fail(c, SGDLM_ERR_ARG, "nsim=%zu out of range", nsim);
+  if (!c->scratch)
+    return fail(c, SGDLM_ERR_DEVICE, "the scratch array of this device has no address");
+  const size_t N = nsim, m = c->m, mp = c->max_p, P = std::max<size_t>((size_t)c->S.P, 1);
+  const size_t chunks = (size_t)vb_chunks((long)N);
+  /* carve the buffers (256-byte aligned) in a fixed order */
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_lam = take(N * m * 8), o_theta = take(N * P * 8), o_ld = take(N * 8),
+               o_w = take(N * 8), o_stats = take(8 * 8), o_flags = take((N + 1) * 4),
+               o_p1 = take(chunks * m * 8), o_p2 = take(chunks * m * (mp + 1) * (mp + 1) * 8),
+               o_ml = take(m * 8), o_mll = take(m * 8), o_mm = take(m * mp * 8),
+               o_mC = take(m * mp * mp * 8), o_mn = take(m * 8), o_ms = take(m * 8),
+               o_mQ = take(m * 8), o_it = take(m * 4);
+  LuPlan plan;
+  rc = lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, c->lu_threads, c->lu_nb, &plan);
+  if (rc)
+    return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
+  const size_t lu_room = off < kScratchBytes ? kScratchBytes - off : 0;
+  const bool fits = lu_room > 0 && lu_plan_cap(c->S, lu_room, &plan) == SGDLM_OK;
+  const size_t o_work = take(plan.work_bytes);
+  if (scratch_bytes)
+    *scratch_bytes = off;
+  if (!fits || off > kScratchBytes)
+    return fail(c, SGDLM_ERR_ARG,
+                "nsim=%zu at m=%zu, max_p=%zu needs more than the %zu bytes of scratch of "
+                "the allocation-free entry point (draws and moments alone take %zu)",
+                nsim, m, mp, kScratchBytes, o_work);
+  char *base = static_cast<char *>(c->scratch);
+  VbDev D;
+  D.lam = reinterpret_cast<double *>(base + o_lam);
+  D.theta = reinterpret_cast<double *>(base + o_theta);
+  D.logdet = reinterpret_cast<double *>(base + o_ld);
+  D.lu_flags = reinterpret_cast<int *>(base + o_flags);
+  D.lu_work = base + o_work;
+  VbBuffers &B = D.B;
+  B.w = reinterpret_cast<double *>(base + o_w);
+  B.stats = reinterpret_cast<double *>(base + o_stats);
+  B.part1 = reinterpret_cast<double *>(base + o_p1);
+  B.part2 = reinterpret_cast<double *>(base + o_p2);
+  B.mean_lambda = reinterpret_cast<double *>(base + o_ml);
+  B.mean_log_lambda = reinterpret_cast<double *>(base + o_mll);
+  B.mean_m = reinterpret_cast<double *>(base + o_mm);
+  B.mean_C = reinterpret_cast<double *>(base + o_mC);
+  B.mean_n = reinterpret_cast<double *>(base + o_mn);
+  B.mean_s = reinterpret_cast<double *>(base + o_ms);
+  B.mean_Q = reinterpret_cast<double *>(base + o_mQ);
+  B.newton_iter = reinterpret_cast<int *>(base + o_it);
+  B.chunks = (int)chunks;
+  if ((rc = run_vb_stages(c, (long)N, D, plan, false)))
+    return rc;
+  if ((mean_m && (rc = d2h(c, mean_m, B.mean_m, m * mp * 8))) ||
+      (mean_C && (rc = d2h(c, mean_C, B.mean_C, m * mp * mp * 8))) ||
+      (mean_n && (rc = d2h(c, mean_n, B.mean_n, m * 8))) ||
+      (mean_s && (rc = d2h(c, mean_s, B.mean_s, m * 8))) ||
+      (w && (rc = d2h(c, w, B.w, N * 8))))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if (sum_det)
     *sum_det = std::exp(c->h_stats[0]) * c->h_stats[1];
   return vb_status(c);

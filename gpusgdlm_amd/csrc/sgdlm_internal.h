@@ -112,6 +112,10 @@ struct LuPlan {
 int lu_dense_cap(size_t lds_max);
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             int want_threads, int want_nb, LuPlan *plan);
+/* Shrinks the persistent grids of a plan (the kernels loop over draws, so
+ * fewer workgroups change only the speed) until its workspace fits in
+ * work_cap bytes; SGDLM_ERR_ARG if not even one workgroup fits. */
+int lu_plan_cap(const Structure &S, size_t work_cap, LuPlan *plan);
 /* LDS layout of the register-resident LU for a dense block of MT 16 x 16
  * tiles: staged panel tiles (bytes, k_lu_rs.hip), and the largest number
  * of tiles per dimension it handles. */

@@ -146,6 +146,29 @@ int sgdlm_get_vb_result(const sgdlm_ctx *ctx, double *mean_m_t,
                         double *mean_C_t, double *mean_n_t, double *mean_s_t,
                         double *IS_weights, double *log_weights, double *ess,
                         double *log_sum_det);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): sgdlm_compute_vb_posterior with nsim
+ * draws whose every device buffer beyond the model state (draws,
+ * log-determinants, LU workspace, weights, moment partials, VB results)
+ * is carved from a static 64 MiB array of this library on the context's
+ * GPU instead of the simulation pool; it calls no hipMalloc / hipFree and
+ * needs no sgdlm_init_sim_memory.  Results go to the caller's host buffers
+ * (NULL skips one) as in sgdlm_compute_vb_posterior, and are identical to
+ * its results for the same RNG state.  Sizes whose buffers exceed the array
+ * fail with SGDLM_ERR_ARG (a draw takes 8 (m + sum_j p_j) + 20 bytes, plus
+ * the LU workspace: m = 6 allows nsim up to ~3e5, m = 100 with
+ * k ~ U{0..5} ~1.2e4);
+ * scratch_bytes (optional) receives the bytes the call needs.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other threads, other contexts) race.  The LU grid may
+ * be narrowed to fit, so large problems run slower than through the
+ * simulation pool. */
+int sgdlm_compute_vb_posterior_noalloc(sgdlm_ctx *ctx, size_t nsim,
+                                       double *mean_m_t, double *mean_C_t,
+                                       double *mean_n_t, double *mean_s_t,
+                                       double *IS_weights,
+                                       double *sum_det_weights,
+                                       size_t *scratch_bytes);
 /* Batched VB for many independent models (one context each, all on this
  * GPU): runs sgdlm_compute_vb_posterior with device-resident results on
  * ctxs[0..count), enqueueing every context's stages on its own stream before
