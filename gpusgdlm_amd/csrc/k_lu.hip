@@ -1564,7 +1564,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
    * register-resident kernel, unless a fast-kernel shape was pinned */
   plan->rs_MT = 0;
   if (!solve && prec != SGDLM_PREC_FP64 && !want_threads && !want_nb && S.e_rb_rows > 0) {
-    plan->rs_MT = M2 / 16;
+    plan->rs_MT = S.e_rb_tiles;
     plan->rs_grid = plan->grid;
     plan->rs_lds = S.e_rb_lds;
     plan->rs_panel_base = S.e_rb_panel_base;

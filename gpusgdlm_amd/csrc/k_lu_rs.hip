@@ -186,19 +186,20 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
   float x[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    float rw[16];
+    /* row r of T four entries at a time (few live registers) */
+    float s = 0.f, dgv = 1.f;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q <= r / 4; ++q) {
       const f32x4 t = lds_load4(T + r * TR + 4 * q);
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        rw[4 * q + e] = t[e];
+      for (int e = 0; e < 4; ++e) {
+        if (4 * q + e < r)
+          s = fmaf(t[e], x[4 * q + e], s);
+        else if (4 * q + e == r)
+          dgv = t[e];
+      }
     }
-    float s = 0.f;
-#pragma unroll
-    for (int t = 0; t < r; ++t)
-      s = fmaf(rw[t], x[t], s);
-    const float rdg = upper ? __builtin_amdgcn_rcpf(rw[r]) : 1.f;
+    const float rdg = upper ? __builtin_amdgcn_rcpf(dgv) : 1.f;
     x[r] = (r == i) ? rdg : (r < i ? 0.f : -s * rdg);
   }
   if (lane < 16) {
@@ -214,7 +215,7 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
   }
 }
 
-template <int WR, int WC, int RT, int CT, int MODE>
+template <int WR, int WC, int RT, int CT, int MODE, int MTT>
 __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a) {
   constexpr int NW = WR * WC, NT = 64 * NW;
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -227,7 +228,9 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a)
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l16 = lane & 15, g = lane >> 4;
   const int wr = wave % WR, wc = wave / WR;
-  const int MT = a.MT;
+  /* MTT > 0: the dense block is padded to exactly MTT tiles (identity), so
+   * the bound checks on tile indices fold at compile time */
+  const int MT = MTT > 0 ? MTT : a.MT;
 
   float *sl = reinterpret_cast<float *>(smem + a.pre_base);
   int *tb = reinterpret_cast<int *>(smem + a.pre_base + a.slot_bytes);
@@ -532,17 +535,23 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a)
 size_t rs_panel_bytes(int MT) { return (size_t)(3 * MT + 4) * TSF * sizeof(float) + 64 * MT; }
 
 /* The instantiation that serves MT tiles per dimension: wavefront grid
- * WR x WC (eight wavefronts, two per SIMD, 256 registers each) and
- * accumulator tiles RT x CT per wavefront. */
+ * WR x WC (eight wavefronts, two per SIMD, 256 registers each),
+ * accumulator tiles RT x CT per wavefront, and the padded tile count when
+ * the kernel is compiled for a fixed one (0: MT at run time). */
 struct RsShape {
-  int WR, WC, RT, CT;
+  int WR, WC, RT, CT, MTT;
 };
 static RsShape rs_shape(int MT) {
   if (MT <= 12)
-    return {2, 4, 6, 3};
+    return {2, 4, 6, 3, 0};
   if (MT <= 16)
-    return {2, 4, 8, 4};
-  return {2, 4, 9, 5};
+    return {2, 4, 8, 4, 0};
+  return {2, 4, 9, 5, 18};
+}
+
+int rs_padded_tiles(int MT) {
+  const RsShape q = rs_shape(MT);
+  return q.MTT > 0 ? q.MTT : MT;
 }
 
 int rs_stage_stride(int MT) {
@@ -594,11 +603,11 @@ int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *t
   if (a.MT > kRsMaxTiles)
     return SGDLM_ERR_ARG;
   if (q.RT == 6)
-    kern = bf ? k_lu_rs<2, 4, 6, 3, RS_BF16X3> : k_lu_rs<2, 4, 6, 3, RS_FP32>;
+    kern = bf ? k_lu_rs<2, 4, 6, 3, RS_BF16X3, 0> : k_lu_rs<2, 4, 6, 3, RS_FP32, 0>;
   else if (q.RT == 8)
-    kern = bf ? k_lu_rs<2, 4, 8, 4, RS_BF16X3> : k_lu_rs<2, 4, 8, 4, RS_FP32>;
+    kern = bf ? k_lu_rs<2, 4, 8, 4, RS_BF16X3, 0> : k_lu_rs<2, 4, 8, 4, RS_FP32, 0>;
   else
-    kern = bf ? k_lu_rs<2, 4, 9, 5, RS_BF16X3> : k_lu_rs<2, 4, 9, 5, RS_FP32>;
+    kern = bf ? k_lu_rs<2, 4, 9, 5, RS_BF16X3, 18> : k_lu_rs<2, 4, 9, 5, RS_FP32, 18>;
   const int threads = 64 * q.WR * q.WC;
   if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)plan.rs_lds) != hipSuccess)

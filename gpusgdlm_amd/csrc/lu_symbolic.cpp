@@ -288,11 +288,11 @@ void build_elim_plan(int m, const std::vector<GammaEntry> &ents, int max_front,
   }
 }
 
-void build_band_plan(const ElimPlan &P, int BR, int stride, BandPlan &B) {
+void build_band_plan(const ElimPlan &P, int BR, int stride, int rows, BandPlan &B) {
   B = BandPlan();
   B.BR = BR;
   B.stride = stride;
-  B.n_band = P.M2 > 0 ? (P.M2 + BR - 1) / BR : 0;
+  B.n_band = rows > 0 ? (rows + BR - 1) / BR : 0;
   const int *brounds = P.tab.data() + P.o_brounds, *bcbase = P.tab.data() + P.o_bcbase;
   /* per band: (pair list, band-local offset) of every Schur target */
   std::vector<std::vector<std::pair<std::vector<uint32_t>, int>>> tg(B.n_band);

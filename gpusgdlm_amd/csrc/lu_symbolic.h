@@ -111,8 +111,9 @@ struct BandPlan {
 };
 
 /* Regroups P's Schur tables into bands of BR rows (BR a multiple of 16)
- * with LDS row stride `stride` (>= P.M2). */
-void build_band_plan(const ElimPlan &P, int BR, int stride, BandPlan &B);
+ * with LDS row stride `stride` (>= rows); `rows` >= P.M2 is the dense
+ * dimension the kernel factors (identity padding beyond P.n2). */
+void build_band_plan(const ElimPlan &P, int BR, int stride, int rows, BandPlan &B);
 
 /* Builds the plan.  max_front = largest degree a prefix pivot may have
  * (0 disables the prefix: n1 = 0, S = the whole matrix); max_slots bounds

@@ -342,9 +342,9 @@ template <typename V> int upload(sgdlm_ctx *c, const char *name, const V &v) {
 int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
   Structure &S = c->S;
   S.e_rb_rows = 0;
-  const int MT = P.M2 / 16;
-  if (MT < 6 || MT > kRsMaxTiles)
+  if (P.M2 / 16 < 6 || P.M2 / 16 > kRsMaxTiles)
     return SGDLM_OK;
+  const int MT = rs_padded_tiles(P.M2 / 16);
   const size_t pre = (((size_t)P.ns * 4 + 15) & ~(size_t)15) + P.tab.size() * 4;
   const size_t flag_base = (pre + 15) & ~(size_t)15;
   const size_t panel_base = flag_base + 16 + 8 * sizeof(double);
@@ -357,7 +357,7 @@ int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
   if (BR == 0)
     return SGDLM_OK;
   BandPlan B;
-  build_band_plan(P, BR, stride, B);
+  build_band_plan(P, BR, stride, 16 * MT, B);
   int rc;
   if ((rc = upload(c, "rb_blk_ptr", B.blk_ptr)) || (rc = upload(c, "rb_st_off", B.st_off)) ||
       (rc = upload(c, "rb_pairs", B.st_pairs)) || (rc = upload(c, "rb_brounds", B.brounds)) ||
@@ -379,6 +379,7 @@ int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
   S.e_rb_panel_base = panel_base;
   S.e_rb_lds = panel_base + std::max(panel, ((size_t)BR * stride * 4 + 15) & ~(size_t)15);
   S.e_rb_rows = BR;
+  S.e_rb_tiles = MT;
   return SGDLM_OK;
 }
 

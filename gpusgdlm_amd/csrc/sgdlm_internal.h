@@ -42,7 +42,7 @@ struct Structure {
   /* The Schur tables regrouped by LDS staging band for the
    * register-resident LU (lu_symbolic.h BandPlan); e_rb_rows = 0: that
    * kernel does not fit this structure. */
-  int e_rb_rows = 0, e_rb_stride = 0, e_rb_nband = 0;
+  int e_rb_rows = 0, e_rb_stride = 0, e_rb_nband = 0, e_rb_tiles = 0;
   size_t e_rb_lds = 0, e_rb_panel_base = 0, e_rb_flag_base = 0;
   const int *e_rb_blk_ptr = nullptr, *e_rb_st_off = nullptr, *e_rb_brounds = nullptr,
             *e_rb_bcbase = nullptr, *e_rb_sg_ptr = nullptr, *e_rb_sg_off = nullptr,
@@ -120,8 +120,10 @@ int lu_plan_cap(const Structure &S, size_t work_cap, LuPlan *plan);
  * tiles: staged panel tiles (bytes, k_lu_rs.hip), and the largest number
  * of tiles per dimension it handles. */
 size_t rs_panel_bytes(int MT);
-/* row stride (floats) of the register-resident LU's LDS staging band */
+/* row stride (floats) of the register-resident LU's LDS staging band, and
+ * the tile count it pads a dense block of MT tiles to (identity) */
 int rs_stage_stride(int MT);
+int rs_padded_tiles(int MT);
 constexpr int kRsMaxTiles = 18;
 /* Register-resident LU log-determinants (plan.rs_MT > 0); flagged draws are
  * appended to flags for the partial-pivoting kernel. */
