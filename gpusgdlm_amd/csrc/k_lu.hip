@@ -144,6 +144,12 @@ struct LuArgs {
   long e_slot_stride;
   const int *e_slot_init, *e_tab, *e_sg_off, *e_sg_idx, *e_st_off;
   const unsigned *e_st_pairs;
+  /* partial-pivoting kernel, log-determinants: pv_schur_M > 0 = re-eliminate
+   * the sparse prefix (LDS at e_pre_base, bound kPrefixLooseMultiplier) and
+   * factor only the pv_schur_M x pv_schur_M Schur complement with row
+   * interchanges; pv_flag_off = LDS byte offset of the prefix flag, outside
+   * the prefix region */
+  int pv_schur_M, pv_flag_off;
   /* forecast solves through the prefix (fast kernel, SOLVE) */
   int e_n2, e_xs_base;
   const int *e_wave_q, *e_node_of, *e_fwd_ptr, *e_bwd_ptr;
@@ -176,11 +182,12 @@ __device__ void assemble(T *A, int M, const double *th, const LuArgs &a, int tid
  * I - Gamma; every entry the prefix updates is then overwritten with
  * -sum x * y over its slot pairs, read in ELL blocks of 64 (one per lane),
  * several blocks in flight per wavefront.  Accumulates the prefix pivots'
- * log|a_kk| (FP64), the zero-pivot flag and max |l|. */
+ * log|a_kk| (FP64), the zero-pivot / multiplier-bound (lbound) flag and
+ * max |l|. */
 template <typename T>
 __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const LuArgs &a,
                                 int tid, int nthreads, bool load_tab, int &bad, T &amax,
-                                double &ls, int *eflag) {
+                                double &ls, int *eflag, T lbound) {
   constexpr unsigned LO = 0xffffu;
   const int NWV = nthreads >> 6; /* wavefronts of the workgroup */
   constexpr int UB = 4;   /* Schur blocks in flight per wavefront */
@@ -199,7 +206,7 @@ __device__ void prefix_assemble(T *A, int M, char *smem, const double *th, const
   const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
   for (size_t q = tid; q < nvec; q += nthreads)
     A4[q] = z;
-  prefix_eliminate<T>(sl, tb, th, pt, tid, nthreads, load_tab, bad, amax, ls, eflag);
+  prefix_eliminate<T>(sl, tb, th, pt, tid, nthreads, load_tab, bad, amax, ls, eflag, lbound);
   for (int r = tid; r < M; r += nthreads)
     A[(size_t)r * M + r] = (T)1;
   for (int e0 = tid; e0 < a.e_nsg; e0 += GB * nthreads) {
@@ -568,7 +575,8 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     if (tid == 0)
       *eflag = 0;
     prefix_assemble<T>(A, M, smem, a.theta + (size_t)d * a.P, a, tid, NT,
-                       !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls, eflag);
+                       !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls, eflag,
+                       (T)kMaxMultiplier);
     bool stop = *eflag != 0;
     if constexpr (SOLVE) {
       if (a.e_slot_save) {
@@ -993,18 +1001,19 @@ __global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a) {
   constexpr int NQ = NB / 16;
   extern __shared__ __attribute__((aligned(16))) char smem[];
 
-  const int M = a.M;
+  const int Mfull = a.M;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int l16 = lane & 15;
   const int g = lane >> 4;
 
+  /* LDS laid out for the full dimension; the Schur mode uses a prefix of it */
   T *Pn = reinterpret_cast<T *>(smem);
-  T *Linv = Pn + (size_t)M * PS;
+  T *Linv = Pn + (size_t)Mfull * PS;
   int *rowmap = reinterpret_cast<int *>(Linv + NB * 16);
-  size_t off = (size_t)M * PS * sizeof(T) + (size_t)NB * 16 * sizeof(T) +
-               (size_t)M * sizeof(int);
+  size_t off = (size_t)Mfull * PS * sizeof(T) + (size_t)NB * 16 * sizeof(T) +
+               (size_t)Mfull * sizeof(int);
   off = (off + 15) & ~(size_t)15;
   double *redv = reinterpret_cast<double *>(smem + off);
   int *redi = reinterpret_cast<int *>(redv + 4);
@@ -1012,18 +1021,50 @@ __global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a) {
   off = (off + 15) & ~(size_t)15;
   double *sv = reinterpret_cast<double *>(smem + off); /* SOLVE only */
 
-  T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * M * M;
+  T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * Mfull * Mfull;
   const long count = a.flag_list ? (long)*a.flag_count : a.n_draws;
 
   for (long q = blockIdx.x; q < count; q += gridDim.x) {
     const long d = a.flag_list ? (long)a.flag_list[q] : q;
     const double *th = a.theta + (size_t)d * a.P;
-    assemble<T>(A, M, th, a, tid, WG);
+    /* the dimension factored for this draw: the Schur complement after the
+     * sparse prefix (log-determinants, prefix within the loose bound) or
+     * the whole matrix */
+    int M = Mfull;
+    double logdet = 0.0;
+    bool schur = false;
+    if constexpr (!SOLVE) {
+      if (a.pv_schur_M > 0) {
+        int *pflag = reinterpret_cast<int *>(smem + a.pv_flag_off);
+        if (tid == 0)
+          *pflag = 0;
+        __syncthreads();
+        int pbad = 0;
+        T pmax = (T)0;
+        double ls = 0.0;
+        prefix_assemble<T>(A, a.pv_schur_M, smem, th, a, tid, WG, true, pbad, pmax, ls, pflag,
+                           (T)kPrefixLooseMultiplier);
+        schur = *pflag == 0;
+        if (schur) {
+          /* prefix log|a_kk| summed over the workgroup in a fixed order */
+#pragma unroll
+          for (int o = 32; o >= 1; o >>= 1)
+            ls += __shfl_xor(ls, o);
+          if (lane == 0)
+            redv[wave] = ls;
+          __syncthreads();
+          logdet = (redv[0] + redv[1]) + (redv[2] + redv[3]);
+          M = a.pv_schur_M;
+        }
+        __syncthreads();
+      }
+    }
+    if (!schur)
+      assemble<T>(A, M, th, a, tid, WG);
     for (int r = tid; r < M; r += WG)
       rowmap[r] = r;
     __syncthreads();
 
-    double logdet = 0.0;
     int singular = 0;
 
     for (int K = 0; K < M; K += NB) {
@@ -1406,6 +1447,21 @@ LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
   return a;
 }
 
+/* The partial-pivoting kernel's view of the arguments (log-determinants):
+ * the whole matrix, and in the Schur mode the prefix region at LDS offset 0
+ * with its tables loaded on every draw. */
+void set_pivot_layout(LuArgs &a, const Structure &S, const LuPlan &plan) {
+  a.M = S.M;
+  a.pv_schur_M = plan.pivot_schur_M;
+  a.pv_flag_off = (int)plan.pivot_flag_off;
+  if (plan.pivot_schur_M > 0) {
+    a.e_pre_base = 0;
+    a.e_tab_persist = 0;
+    a.e_slot_bytes =
+        (int)((((size_t)S.e_ns * (plan.prec == SGDLM_PREC_FP64 ? 8 : 4)) + 15) & ~(size_t)15);
+  }
+}
+
 void set_fast_layout(LuArgs &a, const Structure &S, const LuPlan &plan) {
   a.M = plan.fast_M;
   a.e_pre_base = (int)plan.pre_base;
@@ -1545,12 +1601,20 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
    * fits in half a CU (its per-column pivot searches are latency-bound),
    * else one with the widest panel that fits */
   plan->pivot_grid = plan->grid;
+  /* log-determinants with a sparse prefix: the pivoting kernel re-eliminates
+   * the prefix in LDS (aliasing its panel region; the prefix flag after both)
+   * and factors only the Schur complement */
+  const bool pv_schur = !solve && S.e_n1 > 0 && M2 < M;
+  plan->pivot_schur_M = 0;
+  plan->pivot_flag_off = 0;
   for (int per_cu = 2; per_cu >= 1 && !plan->pivot_nb; --per_cu) {
     for (int i = 0; i < 3 && !plan->pivot_nb; ++i) {
       const int nb = cands[i];
       if ((prec == SGDLM_PREC_FP64 || per_cu == 2) && nb > 32)
         continue;
-      const size_t lds = pivot_lds_bytes(M, nb, ts, solve != 0);
+      size_t lds = pivot_lds_bytes(M, nb, ts, solve != 0);
+      if (pv_schur)
+        lds = ((std::max(lds, pre_bytes) + 15) & ~(size_t)15) + 16;
       if (lds <= lds_max / per_cu) {
         plan->pivot_nb = nb;
         plan->pivot_lds = lds;
@@ -1560,6 +1624,10 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   }
   if (plan->pivot_nb == 0)
     return SGDLM_ERR_ARG;
+  if (pv_schur) {
+    plan->pivot_schur_M = M2;
+    plan->pivot_flag_off = plan->pivot_lds - 16;
+  }
   /* FP32 / BF16x3 log-determinants of dense blocks up to 288: the
    * register-resident kernel, unless a fast-kernel shape was pinned */
   plan->rs_MT = 0;
@@ -1615,6 +1683,7 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
     a.flag_list = nullptr;
     if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
       return SGDLM_ERR_DEVICE;
+    set_pivot_layout(a, S, plan);
     return dispatch_pivot<false>(a, plan, st);
   }
   if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
@@ -1628,7 +1697,7 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   }
   if (rc)
     return rc;
-  a.M = S.M;
+  set_pivot_layout(a, S, plan);
   return dispatch_pivot<false>(a, plan, st);
 }
 

@@ -265,7 +265,7 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a)
       *flagB = 0;
     }
     prefix_eliminate<float>(sl, tb, th, a.pre, tid, NT, !a.tab_persist || d == (long)blockIdx.x, bad,
-                            amax, ls, flagA);
+                            amax, ls, flagA, kMaxMultiplier);
     bool stop = *flagA != 0;
 
     f32x4 acc[RT][CT];

@@ -24,6 +24,15 @@ namespace sgdlm {
  * partial pivoting by k_lu_pivot. */
 constexpr float kMaxMultiplier = 2.0f;
 
+/* Multiplier bound of the sparse prefix when the partial-pivoting kernel
+ * re-eliminates it for a flagged draw and then factors only the dense
+ * Schur complement with row interchanges.  The prefix pivots are the unit
+ * diagonal of I - Gamma modified by at most a few low-degree eliminations;
+ * with |l| <= 64 the log-determinant matches that of partial pivoting on
+ * the whole matrix to within the conditioning of the draw.  Beyond it (or
+ * at a zero prefix pivot) the whole matrix is factored with interchanges. */
+constexpr float kPrefixLooseMultiplier = 64.0f;
+
 /* The prefix part of the plan as the kernels read it. */
 struct PrefixTabs {
   int ns, waves, n1, tab_words;
@@ -36,12 +45,12 @@ struct PrefixTabs {
  * wave: per wave the multipliers l_ik = a_ik / a_kk replace a_ik, then every
  * target slot subtracts its gathered sum of l_ik * a_kj (fixed order, no
  * atomics).  Adds log|a_kk| of the prefix pivots to ls (FP64); a zero pivot
- * or a multiplier beyond kMaxMultiplier (NaN included) sets bad and *eflag;
- * amax tracks max |l|.  Ends with a workgroup barrier. */
+ * or a multiplier beyond lbound (NaN included) sets bad and *eflag; amax
+ * tracks max |l|.  Ends with a workgroup barrier. */
 template <typename T>
 __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixTabs &p, int tid,
                                  int nthreads, bool load_tab, int &bad, T &amax, double &ls,
-                                 int *eflag) {
+                                 int *eflag, T lbound) {
   constexpr unsigned LO = 0xffffu;
   /* the global reads below are issued GB at a time per thread: one round
    * trip per batch instead of one per element */
@@ -100,7 +109,7 @@ __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixT
       sl[x & LO] = l;
       /* a multiplier beyond the bound, or NaN (which fmax would drop),
        * flags the draw through bad as well as the early-exit flag */
-      if (!(fabs(l) <= (T)kMaxMultiplier))
+      if (!(fabs(l) <= lbound))
         bad = 1;
       amax = fmax(amax, fabs(l));
       if (bad)

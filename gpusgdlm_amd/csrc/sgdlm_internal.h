@@ -99,6 +99,11 @@ struct LuPlan {
   size_t slot_save_off = 0;   /* byte offset in the workspace, 0 = unused  */
   size_t slot_stride = 0;     /* bytes per workgroup                        */
   size_t pivot_lds = 0;       /* dynamic LDS bytes, pivoting kernel         */
+  /* log-determinants: the pivoting kernel re-eliminates the sparse prefix
+   * and factors the pivot_schur_M x pivot_schur_M Schur complement with row
+   * interchanges (0: the whole matrix); LDS offset of its prefix flag */
+  int pivot_schur_M = 0;
+  size_t pivot_flag_off = 0;
   size_t work_bytes = 0;      /* global workspace bytes                      */
   /* register-resident LU of the dense block (k_lu_rs.hip) instead of the
    * fast kernel: rs_MT > 0 */
