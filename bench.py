@@ -49,8 +49,11 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps, fc_check=0):
     configure(g, mod)
     filter_steps(g, mod, 1, T_filter)
     g.computePosterior(mod.y[T_filter], mod.F(T_filter))
-    for _ in range(warmup):
-        g.vb_device(N, N)
+    t0 = time.perf_counter()
+    for _ in range(max(warmup, 1)):
+        g.vb_device(N, N)          # the first call tunes the LU configuration
+    warm_s = time.perf_counter() - t0
+    tune = g.lu_autotune()
     g.set_timing(True)
     times = []
     for _ in range(steps):
@@ -70,7 +73,8 @@ def run_gpu(mod, prec, N, steps, warmup, T_filter, fc_N, fc_steps, fc_check=0):
            "p99_ms": 1e3 * float(np.percentile(times, 99)),
            "stage_ms": {k: v[0] / max(v[1], 1) for k, v in stage.items()},
            "ess": res["ess"], "lu_fallback_draws": lu_fb,
-           "lu_config": g.lu_config(), "elimination": g.lu_elimination(), "_check": chk}
+           "lu_config": g.lu_config(), "elimination": g.lu_elimination(), "_check": chk,
+           "autotune": tune, "warmup_s": warm_s}
     # PCIe-inclusive figure through the full API (outputs copied to host)
     t1 = time.perf_counter()
     g.computeVBPosterior(N, N)
@@ -116,7 +120,13 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
         filter_steps(g, mod, 1, T_filter)
         g.computePosterior(mod.y[T_filter], mod.F(T_filter))
         gs.append(g)
-    batch_vb(gs, N)
+    t0 = time.perf_counter()
+    batch_vb(gs, N)                    # first call: each structure's LU tuning
+    first_s = time.perf_counter() - t0
+    tunes = [g.lu_autotune() for g in gs]
+    kinds = {}
+    for t in tunes:
+        kinds[t["kernel"]] = kinds.get(t["kernel"], 0) + 1
     t0 = time.perf_counter()
     for _ in range(steps):
         batch_vb(gs, N)
@@ -140,7 +150,10 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
             "sequential_instance_vb_steps_per_s": K * steps / ts,
             "batched_ms_per_step": 1e3 * tb / steps,
             "instance_step_ms_p50": 1e3 * float(np.percentile(per, 50)),
-            "instance_step_ms_p99": 1e3 * float(np.percentile(per, 99))}
+            "instance_step_ms_p99": 1e3 * float(np.percentile(per, 99)),
+            "first_batched_call_s": first_s,
+            "lu_tuning_s_total": 1e-3 * sum(t["tune_ms"] for t in tunes),
+            "lu_tuned_kernels": kinds}
 
 
 def host_cpu():
@@ -159,6 +172,48 @@ def host_cpu():
     except AttributeError:
         avail = os.cpu_count()
     return {"model": name, "logical_cpus": os.cpu_count(), "available_to_process": avail}
+
+
+def lu_ms(mod, prec, N, T_filter, steps=5, setup=None):
+    """Mean LU stage time (HIP events) of `steps` VB steps after one
+    warm-up, with the context configured by setup(g) first."""
+    from gpusgdlm_amd import SGDLM
+    from gpusgdlm_amd.workloads import configure, filter_steps
+    g = SGDLM(strict=True)
+    g.set_rng_state(1234, 0)
+    g.set_precision(prec)
+    configure(g, mod)
+    if setup:
+        setup(g)
+    filter_steps(g, mod, 1, T_filter)
+    g.computePosterior(mod.y[T_filter], mod.F(T_filter))
+    g.vb_device(N, N)
+    g.set_timing(True)
+    for _ in range(steps):
+        g.vb_device(N, N)
+    t = g.get_timing("lu")
+    out = {"lu_ms": t[0] / max(t[1], 1), "lu_fallback_draws": g.lu_fallback_count(),
+           "lu": g.lu_config()}
+    g.close()
+    return out
+
+
+def run_early(m, kmax, N, T_early, T_late, precs):
+    """Early-series LU cost: after T_early naive filter steps the posterior
+    of Gamma is wide and most draws need row interchanges (pivoting kernel
+    on the Schur complement); LU time against the same model after T_late
+    steps."""
+    from gpusgdlm_amd.workloads import random_model
+    mod = random_model(m, kmax, T=T_late + 2, seed=20250101)
+    out = {"m": m, "nsim": N, "T_early": T_early, "T_late": T_late}
+    for prec in precs:
+        e = lu_ms(mod, prec, N, T_early)
+        l = lu_ms(mod, prec, N, T_late)
+        out[prec] = {"early_lu_ms": e["lu_ms"], "late_lu_ms": l["lu_ms"],
+                     "ratio": e["lu_ms"] / l["lu_ms"], "early_fallback_draws": e["lu_fallback_draws"],
+                     "late_fallback_draws": l["lu_fallback_draws"], "pivot_dim": e["lu"]["pivot_dim"]}
+        log(f"[early m={m} {prec}] {out[prec]}")
+    return out
 
 
 def relerr(a, b):
@@ -307,6 +362,8 @@ def main():
                     help="skip the m=1000, k~U{0..30}, N=1e4 precision-stretch leg")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the FP64-oracle error table of the headline workload")
+    ap.add_argument("--no-early", action="store_true",
+                    help="skip the early-series leg (m=400 after 30 vs 250 filter steps, N=1e4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB (counter calibration)")
@@ -374,6 +431,7 @@ def main():
                          "ms_per_step": v["ms_per_step"], "median_ms": v["median_ms"], "p99_ms": v["p99_ms"],
                          "stage_ms": v["stage_ms"], "ess": v["ess"],
                          "lu_fallback_draws": v["lu_fallback_draws"], "lu": v["lu_config"],
+                         "lu_autotune": v["autotune"],
                          "api_steps_per_s": 1e3 / v["api_ms"], "forecast": v.get("forecast")}
                      for p, v in results.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
@@ -400,6 +458,17 @@ def main():
                                f"the array's largest entry, ESS relative, normalised log-weights "
                                f"absolute (oracle VB {dt:.1f} s); forecasts: first 2000 of "
                                f"{a.forecast_nsim} 1-step draws")
+    # the autotuner's pick against the untuned default plan on the headline
+    # workload (LU stage only)
+    if not a.no_errors:
+        d = lu_ms(mod, head, N, a.filter_steps, setup=lambda g: g.set_lu_autotune(False))
+        line["autotune"] = {"tuned": r["autotune"], "tuned_lu_ms": r["stage_ms"]["lu"],
+                            "default_plan_lu_ms": d["lu_ms"], "default_plan": d["lu"],
+                            "tuned_vs_default": r["stage_ms"]["lu"] / d["lu_ms"]}
+        log(f"[autotune] {line['autotune']}")
+    if not a.no_early:
+        line["early_series"] = run_early(a.m, a.kmax, 10000, 30, a.filter_steps,
+                                         [p for p in ("fp32", "fp64") if p in results])
     if not a.no_stretch:
         line["precision_stretch"] = run_stretch(1000, 30, 10000, a.filter_steps, a.precisions.split(","),
                                                 0 if a.no_errors else 1000)

@@ -73,6 +73,11 @@ SIGNATURES = {
     "sgdlm_get_lu_solve_config": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sgdlm_set_lu_tuning": (C.c_int, [_ctx, C.c_int, C.c_int]),
+    "sgdlm_get_lu_kernel": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sgdlm_set_lu_kernel": (C.c_int, [_ctx, C.c_int]),
+    "sgdlm_set_lu_autotune": (C.c_int, [_ctx, C.c_int]),
+    "sgdlm_get_lu_autotune": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sgdlm_set_lu_max_front": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_batch_compute_vb_posterior": (C.c_int, [C.POINTER(_ctx), C.c_size_t, C.POINTER(C.c_int)]),
     "sgdlm_get_lu_elimination": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -1314,6 +1314,19 @@ size_t pivot_lds_bytes(int M, int NB, size_t ts, bool solve) {
   return b;
 }
 
+/* One int set on the device: the fallback counter before an LU launch (a
+ * kernel rather than a memset node, so that a captured graph replays it
+ * like every other launch). */
+__global__ void k_set_int(int *p, int v) {
+  if (threadIdx.x == 0)
+    *p = v;
+}
+
+int set_int(int *p, int v, hipStream_t st) {
+  k_set_int<<<1, 64, 0, st>>>(p, v);
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
 template <typename Kern>
 int launch_kernel(Kern kern, size_t lds, int grid, int threads, const LuArgs &a, hipStream_t st) {
   if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1485,7 +1498,7 @@ int lu_dense_cap(size_t lds_max) {
 }
 
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
-            int want_threads, int want_nb, LuPlan *plan) {
+            int want_threads, int want_nb, LuPlan *plan, int want_kind) {
   const int M = S.M;
   const int M2 = S.e_M2;
   const size_t ts = (prec == SGDLM_PREC_FP64) ? 8 : 4;
@@ -1502,7 +1515,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   /* Fast-kernel configurations in order of preference: 256 threads with two
    * workgroups per CU (LDS within half a CU, M <= 512), then 512 threads
    * with one workgroup per CU. */
-  for (int pass = 0; pass < 2 && !plan->nb && !solve; ++pass) {
+  for (int pass = 0; pass < 2 && !plan->nb && !solve && want_kind != SGDLM_LU_PIVOTING; ++pass) {
     const int threads = pass == 0 ? 256 : 512;
     const int per_cu = pass == 0 ? 2 : 1;
     if (want_threads && want_threads != threads)
@@ -1595,7 +1608,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
       }
     }
   }
-  if (!solve && (want_threads || want_nb) && !plan->nb)
+  if (!solve && (want_threads || want_nb || want_kind == SGDLM_LU_BLOCKED) && !plan->nb)
     return SGDLM_ERR_ARG;
   /* partial pivoting: two workgroups per CU when a 32- or 16-wide panel
    * fits in half a CU (its per-column pivot searches are latency-bound),
@@ -1629,15 +1642,19 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     plan->pivot_flag_off = plan->pivot_lds - 16;
   }
   /* FP32 / BF16x3 log-determinants of dense blocks up to 288: the
-   * register-resident kernel, unless a fast-kernel shape was pinned */
+   * register-resident kernel, unless another kind or a fast-kernel shape
+   * was asked for */
   plan->rs_MT = 0;
-  if (!solve && prec != SGDLM_PREC_FP64 && !want_threads && !want_nb && S.e_rb_rows > 0) {
+  if (!solve && prec != SGDLM_PREC_FP64 && !want_threads && !want_nb &&
+      (want_kind == SGDLM_LU_AUTO || want_kind == SGDLM_LU_REGISTER) && S.e_rb_rows > 0) {
     plan->rs_MT = S.e_rb_tiles;
     plan->rs_grid = plan->grid;
     plan->rs_lds = S.e_rb_lds;
     plan->rs_panel_base = S.e_rb_panel_base;
     plan->rs_flag_base = S.e_rb_flag_base;
   }
+  if (want_kind == SGDLM_LU_REGISTER && !plan->rs_MT)
+    return SGDLM_ERR_ARG;
   plan->work_bytes =
       (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * M * M * ts;
   if (plan->slot_stride) {
@@ -1645,6 +1662,12 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     plan->work_bytes = plan->slot_save_off + (size_t)plan->fast_grid * plan->slot_stride;
   }
   return SGDLM_OK;
+}
+
+int lu_plan_kind(const LuPlan &plan) {
+  if (plan.rs_MT > 0)
+    return SGDLM_LU_REGISTER;
+  return plan.nb ? SGDLM_LU_BLOCKED : SGDLM_LU_PIVOTING;
 }
 
 int lu_plan_cap(const Structure &S, size_t work_cap, LuPlan *plan) {
@@ -1678,15 +1701,16 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   a.flag_count = flags;
   a.flag_list = flags + 1;
   if (plan.nb == 0 && plan.rs_MT == 0) {
-    /* no fast configuration: partial pivoting for every draw */
+    /* no fast configuration: partial pivoting for every draw (all of them
+     * count as fallback draws) */
     a.flag_count = nullptr;
     a.flag_list = nullptr;
-    if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
+    if (set_int(flags, (int)N, st))
       return SGDLM_ERR_DEVICE;
     set_pivot_layout(a, S, plan);
     return dispatch_pivot<false>(a, plan, st);
   }
-  if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
+  if (set_int(flags, 0, st))
     return SGDLM_ERR_DEVICE;
   int rc;
   if (plan.rs_MT > 0) {
@@ -1708,13 +1732,16 @@ int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
   a.rhs = rhs;
   a.y = y;
   a.refine = refine;
-  if (plan.nb == 0)
+  if (plan.nb == 0) {
+    if (flags && set_int(flags, (int)N, st))
+      return SGDLM_ERR_DEVICE;
     return dispatch_pivot<true>(a, plan, st);
+  }
   /* no-interchange fast path with the sparse prefix; draws it flags are
    * solved by the partial-pivoting kernel */
   a.flag_count = flags;
   a.flag_list = flags + 1;
-  if (hipMemsetAsync(flags, 0, sizeof(int), st) != hipSuccess)
+  if (set_int(flags, 0, st))
     return SGDLM_ERR_DEVICE;
   set_fast_layout(a, S, plan);
   int rc = dispatch_fast<true>(a, plan, st);

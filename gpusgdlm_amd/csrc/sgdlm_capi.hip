@@ -16,9 +16,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -77,6 +79,20 @@ struct PendingEvent {
   hipEvent_t a, b;
 };
 
+/* A tuned VB LU configuration: kernel kind, fast-kernel shape (0 for the
+ * register-resident kind) and prefix front limit, with what finding it
+ * cost. */
+struct LuTuned {
+  int kind = SGDLM_LU_AUTO, threads = 0, nb = 0, max_front = -1;
+  double tune_ms = 0.0, best_ms = 0.0;
+  int candidates = 0;
+  bool from_cache = false;
+};
+
+/* Tuned configurations of the process, by structure key. */
+std::mutex g_tune_mu;
+std::map<uint64_t, LuTuned> g_tune;
+
 } // namespace
 
 struct sgdlm_ctx {
@@ -105,10 +121,16 @@ struct sgdlm_ctx {
   bool max_front_fixed = false; /* set by sgdlm_set_lu_max_front          */
   int max_front_used = 0;       /* front limit of the current plan        */
   int lu_threads = 0, lu_nb = 0; /* fast LU kernel tuning (0 = automatic) */
+  int lu_kind = SGDLM_LU_AUTO;  /* pinned LU kernel kind                     */
+  int max_front_default = 0;    /* front limit the default rule chose        */
+  bool autotune = true;
+  uint64_t tuned_key = 0;       /* structure key of `tuned` (0: none)        */
+  LuTuned tuned;
   uint64_t seed = 1234ull;
   uint32_t epoch = 0;
 
   std::vector<unsigned> h_p, h_sp;
+  std::vector<GammaEntry> ents; /* entries of Gamma of the current sets     */
   Structure S;
 
   /* model memory */
@@ -385,15 +407,16 @@ int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
 
 /* Symbolic elimination plan of the VB log-determinant LU for the current
  * parental sets (lu_symbolic.h).  Its LDS slots (FP64) and tables must fit
- * the LDS of one workgroup; the front limit is halved until they do. */
-int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
+ * the LDS of one workgroup; the front limit is halved until they do.
+ * front >= 0 (the autotuner): that limit instead of the context's rule. */
+int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents, int front = -1) {
   ElimPlan P;
   const size_t budget = c->lds_max > 1024 ? c->lds_max - 1024 : 0;
   const int max_slots = (int)std::min<size_t>(65536, budget / 8);
   auto fits = [&](const ElimPlan &Q) {
     return (size_t)Q.ns * 8 + 16 + Q.tab.size() * 4 <= budget;
   };
-  int mf = c->max_front;
+  int mf = front >= 0 ? front : c->max_front;
   for (;; mf /= 2) {
     build_elim_plan((int)c->m, ents, mf, max_slots, P);
     if (fits(P) || mf == 0)
@@ -404,7 +427,7 @@ int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents) {
    * somewhat larger front (a few more prefix eliminations) usually brings
    * it under; that costs far less than the narrower panels would. */
   const int cap = lu_dense_cap(c->lds_max);
-  if (!c->max_front_fixed && mf == c->max_front && P.M2 > cap && P.M2 <= cap + 64) {
+  if (front < 0 && !c->max_front_fixed && mf == c->max_front && P.M2 > cap && P.M2 <= cap + 64) {
     for (int mf2 = mf + 4; mf2 <= 64; mf2 += 4) {
       ElimPlan Q;
       build_elim_plan((int)c->m, ents, mf2, max_slots, Q);
@@ -518,12 +541,71 @@ int build_structure(sgdlm_ctx *c) {
   S.g_col = M_<int>(c, "g_col");
   S.g_idx = M_<int>(c, "g_idx");
   S.g_rowptr = M_<int>(c, "g_rowptr");
-  return build_elimination(c, ents);
+  c->ents = ents;
+  if ((rc = build_elimination(c, ents)))
+    return rc;
+  c->max_front_default = c->max_front_used;
+  return SGDLM_OK;
+}
+
+/* Key of the tuned-configuration table: FNV-1a over what the LU plans
+ * depend on (sizes, parental sets, precision, device limits, front rule). */
+uint64_t structure_key(const sgdlm_ctx *c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  };
+  mix(c->m);
+  mix(c->max_p);
+  mix((uint64_t)c->prec);
+  mix((uint64_t)c->n_cu);
+  mix(c->lds_max);
+  mix((uint64_t)c->max_front);
+  for (size_t j = 0; j < c->h_p.size(); ++j) {
+    mix(c->h_p[j]);
+    for (unsigned k = 0; k < c->h_p[j]; ++k)
+      mix(c->h_sp[j * c->max_p + k]);
+  }
+  return h | 1u; /* never 0 (= no tuned configuration) */
+}
+
+/* A choice pinned by the caller turns the autotuner off. */
+bool lu_pinned(const sgdlm_ctx *c) {
+  return c->lu_threads || c->lu_nb || c->lu_kind != SGDLM_LU_AUTO || c->max_front_fixed;
+}
+
+/* The VB LU plan: pinned choices, else the tuned configuration of the
+ * current structure (when its front limit is the one built), else the
+ * plan's defaults. */
+int vb_lu_plan(sgdlm_ctx *c, LuPlan *plan) {
+  int th = c->lu_threads, nb = c->lu_nb, kind = c->lu_kind;
+  const uint64_t key = structure_key(c);
+  if (!lu_pinned(c) && c->autotune && c->tuned_key != key) {
+    /* tuned by another context of the process: usable as is when its front
+     * limit is the one built (no re-planning here) */
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end() && it->second.max_front == c->max_front_used) {
+      c->tuned = it->second;
+      c->tuned.from_cache = true;
+      c->tuned.tune_ms = 0.0;
+      c->tuned_key = key;
+    }
+  }
+  if (!lu_pinned(c) && c->autotune && c->tuned_key == key &&
+      c->tuned.max_front == c->max_front_used) {
+    th = c->tuned.threads;
+    nb = c->tuned.nb;
+    kind = c->tuned.kind;
+  }
+  return lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, th, nb, plan, kind);
 }
 
 int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
-  int rc = lu_plan(c->S, prec, solve, c->n_cu, c->lds_max, solve ? 0 : c->lu_threads,
-                   solve ? 0 : c->lu_nb, plan);
+  int rc = solve ? lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, plan) : vb_lu_plan(c, plan);
   if (rc)
     return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
   return dalloc(c, c->sim, "lu_work", plan->work_bytes);
@@ -539,9 +621,138 @@ struct VbDev {
   VbBuffers B;
 };
 
+/* Times every feasible VB LU configuration (kernel kind x workgroup size x
+ * panel width, pivoting for every draw, for the default front limit and
+ * 24 / 48) on the first
+ * min(N, 4096) draws of D (two runs each, the faster counts; one for a
+ * candidate 1.5x slower than the best so far) and returns
+ * the fastest.  The draws' log-determinants are recomputed by the caller;
+ * leaves the elimination plan of the last front tried. */
+int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
+  const long nt = std::min<long>(N, 4096);
+  const auto h0 = std::chrono::steady_clock::now();
+  std::vector<int> fronts{c->max_front_default};
+  if (c->max_front > 0)
+    for (int f : {24, 48})
+      if (f != c->max_front_default)
+        fronts.push_back(f);
+  struct Cand {
+    int kind, threads, nb;
+  };
+  std::vector<Cand> cands;
+  if (c->prec != SGDLM_PREC_FP64)
+    cands.push_back({SGDLM_LU_REGISTER, 0, 0});
+  for (int th : {256, 512})
+    for (int nb : {64, 32, 16})
+      cands.push_back({SGDLM_LU_BLOCKED, th, nb});
+  /* always feasible: row interchanges for every draw (the only kind for
+   * dense blocks beyond the fast kernels' LDS) */
+  cands.push_back({SGDLM_LU_PIVOTING, 0, 0});
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(c, SGDLM_ERR_DEVICE, "hipEventCreate failed");
+  }
+  int rc = SGDLM_OK, n = 0;
+  double best_ms = 1e300;
+  for (int mf : fronts) {
+    if (mf != c->max_front_used && (rc = build_elimination(c, c->ents, mf)))
+      break;
+    for (const Cand &k : cands) {
+      LuPlan p;
+      if (lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, k.threads, k.nb, &p, k.kind))
+        continue;
+      if ((rc = dalloc(c, c->sim, "lu_work", p.work_bytes)))
+        break;
+      D.lu_work = c->sim["lu_work"].ptr;
+      float ms = 1e30f;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {
+        float t = 0.f;
+        if (hipEventRecord(e0, c->stream) != hipSuccess ||
+            launch_lu_logdet(c->S, p, nt, D.theta, D.lu_work, D.logdet, D.lu_flags, c->stream) ||
+            hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+          rc = fail(c, SGDLM_ERR_DEVICE, "LU autotuning launch failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+        ms = std::min(ms, t);
+        if (t > 1.5 * best_ms) /* clearly slower: no second run */
+          break;
+      }
+      if (rc)
+        break;
+      ++n;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best->kind = k.kind;
+        best->threads = k.threads;
+        best->nb = k.nb;
+        best->max_front = mf;
+      }
+    }
+    if (rc)
+      break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc)
+    return rc;
+  if (n == 0)
+    return fail(c, SGDLM_ERR_ARG, "no LU configuration fits m=%zu in LDS", c->m);
+  best->best_ms = best_ms;
+  best->candidates = n;
+  best->from_cache = false;
+  best->tune_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+  return SGDLM_OK;
+}
+
+/* Before the LU of a VB call: the tuned configuration of the current
+ * structure from the process table, or tuned now on this call's draws;
+ * rebuilds the elimination plan for its front limit and re-plans. */
+int apply_lu_tuning(sgdlm_ctx *c, long N, VbDev &D, LuPlan &plan) {
+  if (lu_pinned(c) || !c->autotune)
+    return SGDLM_OK;
+  const uint64_t key = structure_key(c);
+  if (c->tuned_key == key && c->tuned.max_front == c->max_front_used)
+    return SGDLM_OK;
+  LuTuned t;
+  bool found = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      t = it->second;
+      found = true;
+    }
+  }
+  int rc;
+  if (found) {
+    t.from_cache = true;
+    t.tune_ms = 0.0;
+  } else {
+    if ((rc = tune_vb_lu(c, N, D, &t)))
+      return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = t;
+  }
+  if (t.max_front != c->max_front_used && (rc = build_elimination(c, c->ents, t.max_front)))
+    return rc;
+  c->tuned = t;
+  c->tuned_key = key;
+  if ((rc = vb_lu_plan(c, &plan)))
+    return fail(c, rc, "the tuned LU configuration does not fit");
+  if ((rc = dalloc(c, c->sim, "lu_work", plan.work_bytes)))
+    return rc;
+  D.lu_work = c->sim["lu_work"].ptr;
+  return SGDLM_OK;
+}
+
 /* The VB stages on the device (no host copies) for N draws; ends with the
- * weight statistics copied to the pinned c->h_stats. */
-int run_vb_stages(sgdlm_ctx *c, long N, const VbDev &D, const LuPlan &plan, bool timed) {
+ * weight statistics copied to the pinned c->h_stats.  tune: the LU
+ * configuration may be tuned on this call's draws (device buffers from the
+ * context's pools only). */
+int run_vb_stages(sgdlm_ctx *c, long N, VbDev D, LuPlan plan, bool timed, bool tune) {
   int rc;
   if ((rc = check_epochs(c, 1)))
     return rc;
@@ -554,6 +765,8 @@ int run_vb_stages(sgdlm_ctx *c, long N, const VbDev &D, const LuPlan &plan, bool
                 M_<double>(c, "s"), c->seed, ep, D.lam, D.theta, c->stream);
   }
   CHECK_LAUNCH(c);
+  if (tune && (rc = apply_lu_tuning(c, N, D, plan)))
+    return rc;
   {
     StageTimer t(c, "lu", t_on);
     rc = launch_lu_logdet(c->S, plan, N, D.theta, D.lu_work, D.logdet, D.lu_flags, c->stream);
@@ -591,7 +804,7 @@ int run_vb(sgdlm_ctx *c) {
   D.lu_flags = SIM<int>(c, "lu_flags");
   D.lu_work = c->sim["lu_work"].ptr;
   D.B = vb_buffers(c);
-  if ((rc = run_vb_stages(c, (long)c->nsim, D, plan, true)))
+  if ((rc = run_vb_stages(c, (long)c->nsim, D, plan, true, true)))
     return rc;
   c->lu_flags_valid = true;
   c->vb_valid = true;
@@ -648,6 +861,12 @@ int sgdlm_create(sgdlm_ctx **out, size_t no_gpus) {
   }
   if (scratch_ptr(c->device, &c->scratch) != hipSuccess)
     c->scratch = nullptr; /* the allocation-free entry point then fails */
+  {
+    /* SGDLM_LU_AUTOTUNE=0: new contexts start with the autotuner off */
+    const char *e = std::getenv("SGDLM_LU_AUTOTUNE");
+    if (e && e[0] == '0')
+      c->autotune = false;
+  }
   if (hipHostMalloc((void **)&c->h_stats, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamDestroy(c->stream);
@@ -1101,7 +1320,7 @@ int sgdlm_compute_vb_posterior_noalloc(sgdlm_ctx *c, size_t nsim, double *mean_m
                o_mC = take(m * mp * mp * 8), o_mn = take(m * 8), o_ms = take(m * 8),
                o_mQ = take(m * 8), o_it = take(m * 4);
   LuPlan plan;
-  rc = lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, c->lu_threads, c->lu_nb, &plan);
+  rc = vb_lu_plan(c, &plan);
   if (rc)
     return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
   const size_t lu_room = off < kScratchBytes ? kScratchBytes - off : 0;
@@ -1135,7 +1354,7 @@ int sgdlm_compute_vb_posterior_noalloc(sgdlm_ctx *c, size_t nsim, double *mean_m
   B.mean_Q = reinterpret_cast<double *>(base + o_mQ);
   B.newton_iter = reinterpret_cast<int *>(base + o_it);
   B.chunks = (int)chunks;
-  if ((rc = run_vb_stages(c, (long)N, D, plan, false)))
+  if ((rc = run_vb_stages(c, (long)N, D, plan, false, false)))
     return rc;
   if ((mean_m && (rc = d2h(c, mean_m, B.mean_m, m * mp * 8))) ||
       (mean_C && (rc = d2h(c, mean_C, B.mean_C, m * mp * mp * 8))) ||
@@ -1381,17 +1600,78 @@ int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, in
   if (!c->parents_set)
     return SGDLM_ERR_STATE;
   LuPlan plan;
-  int rc = lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, c->lu_threads, c->lu_nb, &plan);
+  int rc = vb_lu_plan(c, &plan);
   if (rc)
     return fail(c, rc, "no LU configuration for threads=%d nb=%d", c->lu_threads, c->lu_nb);
+  const bool reg = plan.rs_MT > 0;
   if (grid)
-    *grid = plan.nb ? plan.fast_grid : plan.grid;
+    *grid = reg ? plan.rs_grid : (plan.nb ? plan.fast_grid : plan.grid);
   if (lds)
-    *lds = plan.lds;
+    *lds = reg ? plan.rs_lds : plan.lds;
   if (nb)
-    *nb = plan.nb;
+    *nb = reg ? 16 : plan.nb;
   if (threads)
-    *threads = plan.nb ? plan.threads : 0;
+    *threads = reg ? 512 : (plan.nb ? plan.threads : 0);
+  return SGDLM_OK;
+}
+
+int sgdlm_get_lu_kernel(const sgdlm_ctx *cc, int *kind, int *dense_dim, int *pivot_dim) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (!c->parents_set)
+    return SGDLM_ERR_STATE;
+  LuPlan plan;
+  int rc = vb_lu_plan(c, &plan);
+  if (rc)
+    return fail(c, rc, "no LU configuration fits");
+  if (kind)
+    *kind = lu_plan_kind(plan);
+  if (dense_dim)
+    *dense_dim = plan.rs_MT > 0 ? 16 * plan.rs_MT : (plan.nb ? plan.fast_M : c->S.M);
+  if (pivot_dim)
+    *pivot_dim = plan.pivot_schur_M > 0 ? plan.pivot_schur_M : c->S.M;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_lu_kernel(sgdlm_ctx *c, int kind) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_PIVOTING)
+    return fail(c, SGDLM_ERR_ARG, "unknown LU kernel kind %d", kind);
+  c->lu_kind = kind;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_lu_autotune(sgdlm_ctx *c, int enable) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  c->autotune = enable != 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_get_lu_autotune(const sgdlm_ctx *c, int *kind, int *threads, int *nb, int *max_front,
+                          double *tune_ms, int *candidates, int *from_cache) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  const bool have = c->parents_set && c->tuned_key != 0 && c->tuned_key == structure_key(c);
+  LuTuned t;
+  if (have)
+    t = c->tuned;
+  if (kind)
+    *kind = have ? t.kind : SGDLM_LU_AUTO;
+  if (threads)
+    *threads = t.threads;
+  if (nb)
+    *nb = t.nb;
+  if (max_front)
+    *max_front = have ? t.max_front : -1;
+  if (tune_ms)
+    *tune_ms = t.tune_ms;
+  if (candidates)
+    *candidates = t.candidates;
+  if (from_cache)
+    *from_cache = t.from_cache ? 1 : 0;
   return SGDLM_OK;
 }
 

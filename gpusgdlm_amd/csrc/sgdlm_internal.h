@@ -110,13 +110,17 @@ struct LuPlan {
   int rs_MT = 0, rs_grid = 0;
   size_t rs_lds = 0, rs_panel_base = 0, rs_flag_base = 0;
 };
-/* want_threads / want_nb: 0 = automatic, else the fast kernel's workgroup
- * size (256 or 512) and panel width; an infeasible request fails. */
 /* Largest dense Schur-complement size (multiple of 16) for which the
  * 64-wide fast LU fits: FP32 in half a CU's LDS, FP64 in a whole CU's. */
 int lu_dense_cap(size_t lds_max);
+/* want_threads / want_nb: 0 = automatic, else the fast kernel's workgroup
+ * size (256 or 512) and panel width; want_kind: SGDLM_LU_AUTO or the
+ * kernel kind to use (SGDLM_LU_BLOCKED / _REGISTER / _PIVOTING).  An
+ * infeasible request fails with SGDLM_ERR_ARG. */
 int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
-            int want_threads, int want_nb, LuPlan *plan);
+            int want_threads, int want_nb, LuPlan *plan, int want_kind = SGDLM_LU_AUTO);
+/* SGDLM_LU_* kind of the kernel that factors the dense block under plan. */
+int lu_plan_kind(const LuPlan &plan);
 /* Shrinks the persistent grids of a plan (the kernels loop over draws, so
  * fewer workgroups change only the speed) until its workspace fits in
  * work_cap bytes; SGDLM_ERR_ARG if not even one workgroup fits. */

@@ -364,10 +364,50 @@ class SGDLM:
         self._call("sgdlm_get_lu_fallback_count", C.byref(n))
         return n.value
 
+    LU_KINDS = {0: "auto", 1: "blocked", 2: "register", 3: "pivoting"}
+
     def lu_config(self):
+        """Launch shape of the VB LU kernel for the current structure and
+        precision (with the tuned or pinned choices), and which kernel it is
+        ("register": the dense block resident in registers, nb = 16-wide
+        tiles; "blocked": LDS panels of nb columns; "pivoting": row
+        interchanges for every draw)."""
         g, l, nb, th = C.c_int(), C.c_size_t(), C.c_int(), C.c_int()
         self._call("sgdlm_get_lu_config", C.byref(g), C.byref(l), C.byref(nb), C.byref(th))
-        return {"grid": g.value, "lds": l.value, "nb": nb.value, "threads": th.value}
+        out = {"grid": g.value, "lds": l.value, "nb": nb.value, "threads": th.value}
+        out.update(self.lu_kernel())
+        return out
+
+    def lu_kernel(self):
+        """Kernel kind of the VB LU, the dimension of the dense block it
+        factors and the dimension the partial-pivoting kernel factors for a
+        draw that needs row interchanges."""
+        k, dd, pd = C.c_int(), C.c_int(), C.c_int()
+        self._call("sgdlm_get_lu_kernel", C.byref(k), C.byref(dd), C.byref(pd))
+        return {"kernel": self.LU_KINDS[k.value], "dense_dim": dd.value, "pivot_dim": pd.value}
+
+    def set_lu_kernel(self, kind: str = "auto"):
+        """Pin the VB LU kernel kind ("auto", "blocked", "register", "pivoting")."""
+        inv = {v: k for k, v in self.LU_KINDS.items()}
+        self._call("sgdlm_set_lu_kernel", C.c_int(inv[kind]))
+
+    def set_lu_autotune(self, on: bool = True):
+        """Time the feasible VB LU configurations on the first VB call of a
+        structure and keep the fastest (default on)."""
+        self._call("sgdlm_set_lu_autotune", C.c_int(1 if on else 0))
+
+    def lu_autotune(self):
+        """The tuned configuration of the current structure (kernel None when
+        none is), the tuning time in ms (0 when taken from the process-wide
+        table), the configurations timed and whether it came from the table."""
+        k, th, nb, mf, n, fc = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        self._call("sgdlm_get_lu_autotune", C.byref(k), C.byref(th), C.byref(nb), C.byref(mf),
+                   C.byref(ms), C.byref(n), C.byref(fc))
+        kind = self.LU_KINDS[k.value]
+        return {"kernel": None if kind == "auto" else kind, "threads": th.value, "nb": nb.value,
+                "max_front": mf.value, "tune_ms": ms.value, "candidates": n.value,
+                "from_cache": bool(fc.value)}
 
     def lu_solve_config(self):
         """Configuration of the forecast-solve LU kernels at the current

@@ -45,6 +45,14 @@ extern "C" {
 #define SGDLM_PREC_FP32 1   /* FP32 LU on MFMA, FP64 log-det sums          */
 #define SGDLM_PREC_BF16X3 2 /* BF16x3 split-precision MFMA trailing update */
 
+/* Kernel kinds of the VB log-determinant LU (sgdlm_get_lu_kernel,
+ * sgdlm_set_lu_kernel, sgdlm_get_lu_autotune). */
+#define SGDLM_LU_AUTO 0     /* chosen by the plan or the autotuner          */
+#define SGDLM_LU_BLOCKED 1  /* dense block through LDS panels (16/32/64)    */
+#define SGDLM_LU_REGISTER 2 /* dense block resident in one CU's registers
+                               (FP32 / BF16x3, up to 288 rows)              */
+#define SGDLM_LU_PIVOTING 3 /* every draw with row interchanges             */
+
 typedef struct sgdlm_ctx sgdlm_ctx;
 
 /* HostWrapperFactory<double>::create(no_gpus) -> HostWrapperImpl ctor,
@@ -222,6 +230,41 @@ int sgdlm_set_lu_max_front(sgdlm_ctx *ctx, int max_front);
 int sgdlm_get_lu_elimination(const sgdlm_ctx *ctx, int *max_front,
                              int *n_prefix, int *n_dense, int *n_waves,
                              long *prefix_fma);
+/* The kernel that factors the dense block of the VB log-determinant LU
+ * for the current structure and precision (SGDLM_LU_BLOCKED / _REGISTER /
+ * _PIVOTING), the dimension it factors (the Schur complement after the
+ * sparse prefix, rounded up to 16) and the dimension the partial-pivoting
+ * kernel factors for a draw that needs row interchanges (the Schur
+ * complement when the prefix is re-eliminated, else the whole matrix
+ * rounded up to 16).  No counterpart in the reference, where cuBLAS picks
+ * its getrfBatched kernel (rSGDLM/src/SGDLM/SGDLM.cu:280-281). */
+int sgdlm_get_lu_kernel(const sgdlm_ctx *ctx, int *kind, int *dense_dim,
+                        int *pivot_dim);
+/* Pins the kernel kind of the VB LU (SGDLM_LU_AUTO = plan / autotuner);
+ * a kind that does not fit makes the next VB call fail with
+ * SGDLM_ERR_ARG.  No counterpart in the reference. */
+int sgdlm_set_lu_kernel(sgdlm_ctx *ctx, int kind);
+/* Autotuner of the VB LU (default on): on the first VB call for a parental
+ * structure and precision, every feasible configuration (kernel kind,
+ * workgroup size, panel width, and prefix front limits 24 / default / 48)
+ * is timed on up to 4096 of that call's own draws, and the fastest is kept
+ * for the structure, in a table shared by all contexts of the process
+ * (keyed by a hash of m, max_p, the parental sets, precision and device).
+ * It does not run while sgdlm_set_lu_tuning, sgdlm_set_lu_kernel or
+ * sgdlm_set_lu_max_front pin a choice, nor in the allocation-free entry
+ * point (which uses a configuration already tuned for its structure).
+ * Results do not depend on it beyond rounding; processes that must agree
+ * bit for bit either pin the configuration or turn tuning off (also for
+ * every new context by setting the environment variable
+ * SGDLM_LU_AUTOTUNE=0).  No counterpart in the reference. */
+int sgdlm_set_lu_autotune(sgdlm_ctx *ctx, int enable);
+/* The tuned configuration for the current structure (kind = SGDLM_LU_AUTO
+ * when none is), the milliseconds the tuning took in this process (0 when
+ * taken from the table), the number of configurations timed and whether it
+ * came from the table. */
+int sgdlm_get_lu_autotune(const sgdlm_ctx *ctx, int *kind, int *threads, int *nb,
+                          int *max_front, double *tune_ms, int *candidates,
+                          int *from_cache);
 
 #ifdef __cplusplus
 }

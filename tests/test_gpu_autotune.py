@@ -1,0 +1,107 @@
+"""VB LU kernel kinds and the autotuner (GPU): every kernel kind a caller
+can pin gives the oracle's log-determinants on the same draws, and the
+autotuner's choice (timed on the first VB call of a structure, then taken
+from the process-wide table by any context with that structure) gives the
+same results as the configuration it names.
+
+Bars: FP64 1e-10 absolute on the log-determinants, FP32 / BF16x3 5e-3
+absolute (the LU bars of test_gpu_parity.py)."""
+import numpy as np
+import pytest
+
+from gpusgdlm_amd import SGDLM
+from gpusgdlm_amd.workloads import configure, filter_steps, random_model
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SEED = 1234
+TOL = {"fp64": 1e-10, "fp32": 5e-3, "bf16x3": 5e-3}
+
+
+def _model():
+    return random_model(400, 20, T=45, seed=9)
+
+
+def _oracle_logdets(mod, N, t_post=40):
+    o = orc.OracleSGDLM(seed=SEED, epoch=0)
+    configure(o, mod)
+    filter_steps(o, mod, 1, t_post)
+    o.computePosterior(mod.y[t_post], mod.F(t_post))
+    prm = o.getParameters()
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    return orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
+
+
+def _gpu(mod, prec, N, setup=None, t_post=40):
+    g = SGDLM(strict=True)
+    g.set_rng_state(SEED, 0)
+    g.set_precision(prec)
+    configure(g, mod)
+    if setup:
+        setup(g)
+    filter_steps(g, mod, 1, t_post)
+    g.computePosterior(mod.y[t_post], mod.F(t_post))
+    g.computeVBPosterior(N, N)
+    out = {"lw": g.get_vb_result()["log_weights"], "cfg": g.lu_config(), "tune": g.lu_autotune(),
+           "el": g.lu_elimination()}
+    g.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp64"])
+def test_pinned_kernel_kinds_vs_oracle(hiplib, prec):
+    mod = _model()
+    N = 192
+    ld = _oracle_logdets(mod, N)
+    kinds = ["blocked", "pivoting"] + ([] if prec == "fp64" else ["register"])
+    for kind in kinds:
+        print(f"pinned LU kernel {kind} {prec} m=400 N={N}", flush=True)
+        r = _gpu(mod, prec, N, lambda g: g.set_lu_kernel(kind))
+        assert r["cfg"]["kernel"] == kind
+        assert r["tune"]["kernel"] is None  # pinned: no tuning
+        err = np.max(np.abs(r["lw"] - ld))
+        print(f"  {r['cfg']}: max |err| {err:.2e}", flush=True)
+        assert err <= TOL[prec]
+
+
+def test_register_kernel_refused_for_fp64(hiplib):
+    mod = _model()
+    g = SGDLM(strict=True)
+    g.set_precision("fp64")
+    configure(g, mod)
+    g.set_lu_kernel("register")
+    with pytest.raises(Exception):
+        g.lu_config()
+    g.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_autotune_then_table(hiplib, prec):
+    """First context: tuned on its first VB call (timed candidates); second
+    context, same structure: the table's entry, bit-identical results; both
+    match the oracle, as does the configuration pinned by hand."""
+    mod = _model()
+    N = 4096
+    ld = _oracle_logdets(mod, N)
+    print(f"autotune {prec} m=400 N={N}", flush=True)
+    a = _gpu(mod, prec, N)
+    t = a["tune"]
+    print(f"  tuned {t}, plan {a['cfg']}, elimination {a['el']}", flush=True)
+    assert t["kernel"] in ("register", "blocked") and t["candidates"] >= 3
+    assert a["cfg"]["kernel"] == t["kernel"] and a["el"]["max_front"] == t["max_front"]
+    b = _gpu(mod, prec, N)
+    assert b["tune"]["from_cache"] and b["tune"]["tune_ms"] == 0.0
+    assert {k: b["tune"][k] for k in ("kernel", "threads", "nb", "max_front")} == \
+        {k: t[k] for k in ("kernel", "threads", "nb", "max_front")}
+    assert np.array_equal(a["lw"], b["lw"])
+
+    def pin(g):
+        g.set_lu_max_front(t["max_front"])
+        if t["kernel"] == "register":
+            g.set_lu_kernel("register")
+        else:
+            g.set_lu_tuning(t["threads"], t["nb"])
+    c = _gpu(mod, prec, N, pin)
+    assert np.array_equal(a["lw"], c["lw"])
+    assert np.max(np.abs(a["lw"] - ld)) <= TOL[prec]

@@ -137,7 +137,17 @@ def test_logdet_fp64_m400_vs_oracle(hiplib):
     prm = o.getParameters()
     lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
     ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
-    assert np.max(np.abs(lw - ld)) <= 1e-10
+    # 1e-10 absolute, or 1e-15 kappa for an ill-conditioned draw (early in
+    # the series most draws are factored with row interchanges on the Schur
+    # complement after the sparse prefix, a different elimination order than
+    # the oracle's; two backward-stable LUs then differ by about kappa eps)
+    err = np.abs(lw - ld)
+    hard = np.nonzero(~(err <= 1e-10))[0]
+    assert len(hard) <= N // 20
+    for i in hard:
+        kappa = np.linalg.cond(_dense_I_minus_Gamma(mod, th[:, :, i]))
+        print(f"  draw {i}: kappa {kappa:.2e}, |err| {err[i]:.2e}", flush=True)
+        assert err[i] <= 1e-15 * kappa
     # and one draw against LAPACK (numpy slogdet)
     A = np.eye(mod.m)
     spf = mod.sp_u32.ravel(order="F")
@@ -270,6 +280,7 @@ def _logdets(mod, prec, N, max_front, t_post=8):
     g.computeVBPosterior(N, N)
     lw = g.get_vb_result()["log_weights"]
     el = g.lu_elimination()
+    el["kernel"] = g.lu_kernel()["kernel"]
     fb = g.lu_fallback_count()
     g.close()
     filter_steps(o, mod, 1, t_post)
@@ -303,11 +314,14 @@ def test_front_raised_for_wide_panels(hiplib, prec, tol):
     """With the default front limit, an m = 400 structure whose dense block
     would be 304 at front 32 (above the 288 up to which the 64-wide fast LU
     fits in LDS) is planned with a slightly larger front: dense block <= 288,
-    64-wide panels, and the oracle's log-determinants."""
+    64-wide panels (FP64) or the register-resident kernel (FP32), and the
+    oracle's log-determinants.  The autotuner is off: it may pick another
+    front."""
     mod = random_model(400, 20, T=45, seed=1)
     N = 128
     print(f"front raise m=400 seed 1 {prec} N={N}", flush=True)
     g, o = make_pair(mod, prec)
+    g.set_lu_autotune(False)
     filter_steps(g, mod, 1, 40)
     g.computePosterior(mod.y[40], mod.F(40))
     g.computeVBPosterior(N, N)
@@ -315,7 +329,8 @@ def test_front_raised_for_wide_panels(hiplib, prec, tol):
     el, cfg, fb = g.lu_elimination(), g.lu_config(), g.lu_fallback_count()
     g.close()
     print(f"  {el} {cfg}")
-    assert el["max_front"] > 32 and el["n_dense"] <= 288 and cfg["nb"] == 64
+    assert el["max_front"] > 32 and el["n_dense"] <= 288
+    assert cfg["kernel"] == "register" if prec == "fp32" else cfg["nb"] == 64
     filter_steps(o, mod, 1, 40)
     o.computePosterior(mod.y[40], mod.F(40))
     prm = o.getParameters()
@@ -339,7 +354,9 @@ def test_logdet_sparse_prefix_m1000(hiplib, prec, tol):
     ld = orc.logdet(mod.m, mod.max_p, mod.p, mod.sp_u32, th)
     print(f"  {el}: max |err| {np.max(np.abs(lw - ld)):.3e}, fallback {fb}")
     assert np.max(np.abs(lw - ld)) <= tol
-    assert fb <= N // 8
+    # a dense block beyond the fast kernels' LDS (FP64) is factored with
+    # row interchanges for every draw, all of which then count as fallback
+    assert fb <= N // 8 or el["kernel"] == "pivoting"
 
 
 @pytest.mark.parametrize("prec,cfgs,tol", [

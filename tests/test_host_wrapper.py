@@ -100,12 +100,17 @@ def test_host_wrapper_matches_python_binding(tmp_path):
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     write_input(inp, mod, T, nsim)
     print(f"host_wrapper_driver readme T={T} nsim={nsim}", flush=True)
-    r = subprocess.run([DRIVER, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    # bit-for-bit comparison across two processes: the LU configuration must
+    # be the same on both sides, so neither tunes it by timing
+    env = dict(os.environ, SGDLM_LU_AUTOTUNE="0")
+    r = subprocess.run([DRIVER, str(inp), str(out)], capture_output=True, text=True, timeout=120,
+                       env=env)
     assert r.returncode == 0 and "[SGDLM] ERROR" not in r.stderr, r.stderr
     res = read_output(out, m, mp, nsim)
     assert list(res["meta"]) == [m, mp, nsim, nsim, 1]
 
     g = SGDLM(strict=True)
+    g.set_lu_autotune(False)
     configure(g, mod)
     for t in range(1, T + 1):
         g.computePosterior(mod.y[t], mod.F(t))

@@ -43,6 +43,9 @@ def test_noalloc_matches_allocating_entry_point(hiplib, kind, prec):
     mod = readme_model(T=30) if kind == "readme" else random_model(100, 5, T=30, seed=11)
     g = SGDLM(strict=True)
     g.set_precision(prec)
+    # the first allocating call would otherwise tune the LU configuration
+    # (the allocation-free one never does): both sides on the same plan
+    g.set_lu_autotune(False)
     configure(g, mod)
     filter_steps(g, mod, 1, 20)
     g.computePosterior(mod.y[20], mod.F(20))
