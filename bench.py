@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "VB steps/s + IS draws/s at m=400,N=5e4: FP32-MFMA mixed vs plain FP64"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lu_fast_vb_m400_n50k_traffic.json")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "vb_m400_n50k_lu_counters.json")
 
 
 def log(msg):
@@ -326,20 +326,22 @@ def cpu_baseline(mod, T_filter, N_cap, budget_s=12.0):
     return {"n": n, "dt": dt, "threads": threads, "n1": n1, "dt1": dt1}
 
 
-def lu_traffic(prec, m, N, cfg):
-    """HBM bytes per k_lu_fast launch from the committed counter profile of
-    the same workload and kernel configuration (None if not measured)."""
+def lu_counters(prec, m, N, cfg):
+    """Counter figures of the LU kernel of this workload (L2<->fabric bytes,
+    MFMA busy cycles and executed MFMA flops per launch) from the committed
+    rocprofv3 profile of the same workload and kernel (None if it does not
+    match)."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(COUNTERS_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
     e = t.get("precisions", {}).get(prec)
     if not e or t.get("m") != m or t.get("nsim") != N:
         return None
-    if e.get("threads") != cfg["threads"] or e.get("nb") != cfg["nb"]:
+    if e.get("kind") != cfg.get("kernel") or e.get("dense_dim") != cfg.get("dense_dim"):
         return None
-    return e["fetch_bytes"] + e["write_bytes"]
+    return e
 
 
 def main():
@@ -407,7 +409,7 @@ def main():
     M2 = (el["n_dense"] + 15) // 16 * 16
     executed = (dense_lu_flops(M2) + 2.0 * el["prefix_fma"]) * N
     achieved = dense_eq / lu_s / 1e12
-    traffic = lu_traffic(head, a.m, N, r["lu_config"])
+    cnt = lu_counters(head, a.m, N, r["lu_config"])
     line = {
         "metric": METRIC,
         "value": steps_s,
@@ -435,9 +437,17 @@ def main():
                          "api_steps_per_s": 1e3 / v["api_ms"], "forecast": v.get("forecast")}
                      for p, v in results.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[head], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[head], "traffic": traffic,
-                     "kernel": f"k_lu_fast ({head}); achieved = dense-equivalent (2/3) m^3 x N = "
-                               f"{dense_eq / 1e12:.3f} TFLOP per launch / LU time",
+                     "frac": achieved / PEAK[head],
+                     "traffic": cnt["fetch_bytes"] + cnt["write_bytes"] if cnt else None,
+                     "traffic_kind": "L2<->fabric bytes per LU launch (FETCH_SIZE x2 + WRITE_SIZE, scaled "
+                                     "by a 1 GiB copy; Infinity-Cache hits included, not HBM alone)",
+                     "mfma_util": cnt["mfma_util"] if cnt else None,
+                     "mfma_util_def": "SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMD x 256 CU x GRBM_GUI_ACTIVE / 8 XCD), "
+                                      "profiles/vb_m400_n50k_lu_counters.md",
+                     "mfma_executed_tflops": cnt["mfma_flop"] / lu_s / 1e12 if cnt else None,
+                     "kernel": f"{cnt['kernel'] if cnt else r['lu_config']['kernel']} ({head}); achieved = "
+                               f"dense-equivalent (2/3) m^3 x N = {dense_eq / 1e12:.3f} TFLOP per launch / "
+                               f"LU time (HIP events, bench stream)",
                      "executed_tflops": executed / lu_s / 1e12,
                      "executed_tflop_per_launch": executed / 1e12},
     }
