@@ -131,6 +131,17 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
     for _ in range(steps):
         batch_vb(gs, N)
     tb = time.perf_counter() - t0
+    # graph mode: each context's VB step captured once, then replayed
+    for g in gs:
+        g.set_graph_mode(True)
+    batch_vb(gs, N)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch_vb(gs, N)
+    tg = time.perf_counter() - t0
+    replays = sum(g.graph_stats()["replays"] for g in gs)
+    for g in gs:
+        g.set_graph_mode(False)
     for g in gs:
         g.vb_device(N, N)
     per = []
@@ -149,6 +160,8 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
             "batched_instance_vb_steps_per_s": K * steps / tb,
             "sequential_instance_vb_steps_per_s": K * steps / ts,
             "batched_ms_per_step": 1e3 * tb / steps,
+            "batched_graph_instance_vb_steps_per_s": K * steps / tg,
+            "graph_replays": replays,
             "instance_step_ms_p50": 1e3 * float(np.percentile(per, 50)),
             "instance_step_ms_p99": 1e3 * float(np.percentile(per, 99)),
             "first_batched_call_s": first_s,
@@ -196,6 +209,30 @@ def lu_ms(mod, prec, N, T_filter, steps=5, setup=None):
            "lu": g.lu_config()}
     g.close()
     return out
+
+
+def vb_graph_ms(mod, prec, N, T_filter, steps=10):
+    """Wall time per device-resident VB step in graph mode (the step
+    captured once, then replayed), after two warm-up calls."""
+    from gpusgdlm_amd import SGDLM
+    from gpusgdlm_amd.workloads import configure, filter_steps
+    g = SGDLM(strict=True)
+    g.set_rng_state(1234, 0)
+    g.set_precision(prec)
+    g.set_graph_mode(True)
+    configure(g, mod)
+    filter_steps(g, mod, 1, T_filter)
+    g.computePosterior(mod.y[T_filter], mod.F(T_filter))
+    for _ in range(2):
+        g.vb_device(N, N)
+    t = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        g.vb_device(N, N)
+        t.append(time.perf_counter() - t0)
+    st = g.graph_stats()
+    g.close()
+    return {"ms_per_step": 1e3 * float(np.mean(t)), "median_ms": 1e3 * float(np.median(t)), **st}
 
 
 def run_early(m, kmax, N, T_early, T_late, precs):
@@ -476,6 +513,8 @@ def main():
                             "default_plan_lu_ms": d["lu_ms"], "default_plan": d["lu"],
                             "tuned_vs_default": r["stage_ms"]["lu"] / d["lu_ms"]}
         log(f"[autotune] {line['autotune']}")
+    line["graph_mode"] = {p: vb_graph_ms(mod, p, N, a.filter_steps) for p in results}
+    log(f"[graph] {line['graph_mode']}")
     if not a.no_early:
         line["early_series"] = run_early(a.m, a.kmax, 10000, 30, a.filter_steps,
                                          [p for p in ("fp32", "fp64") if p in results])

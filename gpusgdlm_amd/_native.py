@@ -76,6 +76,8 @@ SIGNATURES = {
     "sgdlm_get_lu_kernel": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sgdlm_set_lu_kernel": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_set_lu_autotune": (C.c_int, [_ctx, C.c_int]),
+    "sgdlm_set_graph_mode": (C.c_int, [_ctx, C.c_int]),
+    "sgdlm_get_graph_stats": (C.c_int, [_ctx, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "sgdlm_get_lu_autotune": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_int), _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sgdlm_set_lu_max_front": (C.c_int, [_ctx, C.c_int]),

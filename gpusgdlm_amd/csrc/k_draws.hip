@@ -86,9 +86,10 @@ __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
                                              long P, const double *m_t,
                                              const double *U, const double *n_t,
                                              const double *s_t, uint64_t seed,
-                                             uint32_t epoch, double *lam,
-                                             double *theta) {
+                                             uint32_t epoch_arg, const uint32_t *epoch_dev,
+                                             double *lam, double *theta) {
   extern __shared__ double sacc[]; /* [max_p][TS] */
+  const uint32_t epoch = epoch_dev ? *epoch_dev : epoch_arg;
   const int j = blockIdx.y;
   const int pj = (int)p[j];
   for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
@@ -112,8 +113,9 @@ __global__ __launch_bounds__(TB) void k_draw_reg(long N, int m, int max_p,
                                                  long P, const double *m_t,
                                                  const double *U, const double *n_t,
                                                  const double *s_t, uint64_t seed,
-                                                 uint32_t epoch, double *lam,
-                                                 double *theta) {
+                                                 uint32_t epoch_arg, const uint32_t *epoch_dev,
+                                                 double *lam, double *theta) {
+  const uint32_t epoch = epoch_dev ? *epoch_dev : epoch_arg;
   const int j = blockIdx.y;
   const int pj = (int)p[j];
   const double *Uj = U + (size_t)j * max_p * max_p, *mj = m_t + (size_t)j * max_p;
@@ -332,17 +334,17 @@ void launch_cholesky(const Structure &S, const double *C, double *U,
 void launch_draw(const Structure &S, long N, const double *m_t,
                  const double *U, const double *n_t, const double *s_t,
                  uint64_t seed, uint32_t epoch, double *lam, double *theta,
-                 hipStream_t st) {
+                 hipStream_t st, const uint32_t *epoch_dev) {
   if (S.max_p <= 32) {
     auto kern = S.max_p <= 8 ? k_draw_reg<8> : S.max_p <= 16 ? k_draw_reg<16>
                 : S.max_p <= 24 ? k_draw_reg<24> : k_draw_reg<32>;
     hipLaunchKernelGGL(kern, grid_nm(N, S.m), dim3(TB), 0, st, N, S.m, S.max_p, S.p, S.off,
-                       S.P, m_t, U, n_t, s_t, seed, epoch, lam, theta);
+                       S.P, m_t, U, n_t, s_t, seed, epoch, epoch_dev, lam, theta);
     return;
   }
   const size_t lds = (size_t)S.max_p * TS * sizeof(double);
   hipLaunchKernelGGL(k_draw, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
-                     S.max_p, S.p, S.off, S.P, m_t, U, n_t, s_t, seed, epoch,
+                     S.max_p, S.p, S.off, S.P, m_t, U, n_t, s_t, seed, epoch, epoch_dev,
                      lam, theta);
 }
 

@@ -1664,6 +1664,8 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   return SGDLM_OK;
 }
 
+int set_device_int(int *p, int v, hipStream_t st) { return set_int(p, v, st); }
+
 int lu_plan_kind(const LuPlan &plan) {
   if (plan.rs_MT > 0)
     return SGDLM_LU_REGISTER;

@@ -89,6 +89,14 @@ struct LuTuned {
   bool from_cache = false;
 };
 
+/* A captured VB step and what it was captured for: replayed only while
+ * every argument it baked in is unchanged. */
+struct VbGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uintptr_t> sig;
+};
+
 /* Tuned configurations of the process, by structure key. */
 std::mutex g_tune_mu;
 std::map<uint64_t, LuTuned> g_tune;
@@ -126,6 +134,11 @@ struct sgdlm_ctx {
   bool autotune = true;
   uint64_t tuned_key = 0;       /* structure key of `tuned` (0: none)        */
   LuTuned tuned;
+  /* graph mode: the VB stages captured once and replayed */
+  bool graph_mode = false;
+  uint64_t gen = 0;             /* bumped whenever device tables are rebuilt */
+  VbGraph vbg;
+  long graph_captures = 0, graph_replays = 0;
   uint64_t seed = 1234ull;
   uint32_t epoch = 0;
 
@@ -482,6 +495,7 @@ int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents, int fro
   S.e_fwd = M_<uint32_t>(c, "e_fwd");
   S.e_bwd_ptr = M_<int>(c, "e_bwd_ptr");
   S.e_bwd = M_<uint32_t>(c, "e_bwd");
+  ++c->gen;
   return build_band_tables(c, P);
 }
 
@@ -752,17 +766,18 @@ int apply_lu_tuning(sgdlm_ctx *c, long N, VbDev &D, LuPlan &plan) {
  * weight statistics copied to the pinned c->h_stats.  tune: the LU
  * configuration may be tuned on this call's draws (device buffers from the
  * context's pools only). */
-int run_vb_stages(sgdlm_ctx *c, long N, VbDev D, LuPlan plan, bool timed, bool tune) {
+/* Enqueues the VB stages for N draws at epoch ep (or the epoch in the
+ * device word ep_dev) on the context's stream; no host synchronisation, no
+ * allocation unless tune (so that a capture can record it).  Ends with the
+ * weight statistics copied to the pinned c->h_stats. */
+int enqueue_vb(sgdlm_ctx *c, long N, VbDev &D, LuPlan &plan, bool t_on, bool tune, uint32_t ep,
+               const uint32_t *ep_dev) {
   int rc;
-  if ((rc = check_epochs(c, 1)))
-    return rc;
-  const uint32_t ep = ++c->epoch;
-  const bool t_on = timed && c->timing;
   {
     StageTimer t(c, "draw", t_on);
     launch_cholesky(c->S, M_<double>(c, "C"), M_<double>(c, "U"), c->stream);
     launch_draw(c->S, N, M_<double>(c, "m"), M_<double>(c, "U"), M_<double>(c, "n"),
-                M_<double>(c, "s"), c->seed, ep, D.lam, D.theta, c->stream);
+                M_<double>(c, "s"), c->seed, ep, D.lam, D.theta, c->stream, ep_dev);
   }
   CHECK_LAUNCH(c);
   if (tune && (rc = apply_lu_tuning(c, N, D, plan)))
@@ -789,6 +804,90 @@ int run_vb_stages(sgdlm_ctx *c, long N, VbDev D, LuPlan plan, bool timed, bool t
   return SGDLM_OK;
 }
 
+int run_vb_stages(sgdlm_ctx *c, long N, VbDev D, LuPlan plan, bool timed, bool tune) {
+  int rc;
+  if ((rc = check_epochs(c, 1)))
+    return rc;
+  const uint32_t ep = ++c->epoch;
+  return enqueue_vb(c, N, D, plan, timed && c->timing, tune, ep, nullptr);
+}
+
+void drop_graph(sgdlm_ctx *c) {
+  if (c->vbg.exec)
+    (void)hipGraphExecDestroy(c->vbg.exec);
+  if (c->vbg.graph)
+    (void)hipGraphDestroy(c->vbg.graph);
+  c->vbg = VbGraph();
+}
+
+/* Everything a captured VB step bakes into its kernel arguments. */
+std::vector<uintptr_t> vb_signature(sgdlm_ctx *c, long N, const VbDev &D, const LuPlan &p) {
+  const VbBuffers &B = D.B;
+  return {(uintptr_t)c->gen, (uintptr_t)c->prec, (uintptr_t)N, (uintptr_t)c->seed,
+          (uintptr_t)(c->seed >> 32), (uintptr_t)D.lam, (uintptr_t)D.theta,
+          (uintptr_t)D.logdet, (uintptr_t)D.lu_flags, (uintptr_t)D.lu_work, (uintptr_t)B.w,
+          (uintptr_t)B.stats, (uintptr_t)B.part1, (uintptr_t)B.part2, (uintptr_t)B.mean_m,
+          (uintptr_t)B.mean_C, (uintptr_t)B.mean_n, (uintptr_t)B.mean_s, (uintptr_t)B.chunks,
+          (uintptr_t)M_<double>(c, "m"), (uintptr_t)M_<double>(c, "C"),
+          (uintptr_t)M_<double>(c, "U"), (uintptr_t)M_<double>(c, "n"),
+          (uintptr_t)M_<double>(c, "s"), (uintptr_t)c->h_stats, (uintptr_t)p.nb,
+          (uintptr_t)p.threads, (uintptr_t)p.fast_grid, (uintptr_t)p.lds, (uintptr_t)p.grid,
+          (uintptr_t)p.rs_MT, (uintptr_t)p.rs_grid, (uintptr_t)p.rs_lds, (uintptr_t)p.pivot_nb,
+          (uintptr_t)p.pivot_grid, (uintptr_t)p.pivot_lds, (uintptr_t)p.pivot_schur_M,
+          (uintptr_t)p.work_bytes};
+}
+
+/* Graph mode: the VB step of run_vb captured once as a hipGraph (re-captured
+ * when anything it baked in changes) and replayed; the epoch reaches the
+ * draw kernels through a device word set by a one-thread kernel first. */
+int run_vb_graph(sgdlm_ctx *c, long N, VbDev D, LuPlan plan) {
+  int rc;
+  if ((rc = check_epochs(c, 1)))
+    return rc;
+  if ((rc = dalloc(c, c->mem, "epoch", 16)))
+    return rc;
+  uint32_t *ep_dev = M_<uint32_t>(c, "epoch");
+  const uint32_t ep = ++c->epoch;
+  if (set_device_int(reinterpret_cast<int *>(ep_dev), (int)ep, c->stream))
+    return fail(c, SGDLM_ERR_DEVICE, "epoch store failed: %s",
+                hipGetErrorString(hipGetLastError()));
+  std::vector<uintptr_t> sig = vb_signature(c, N, D, plan);
+  sig.push_back((uintptr_t)ep_dev);
+  if (!c->vbg.exec || c->vbg.sig != sig) {
+    drop_graph(c);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int erc = enqueue_vb(c, N, D, plan, false, false, ep, ep_dev);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (erc || e != hipSuccess) {
+      if (g)
+        (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return erc ? erc : fail(c, SGDLM_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return fail(c, SGDLM_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    c->vbg.graph = g;
+    c->vbg.exec = x;
+    c->vbg.sig = sig;
+    ++c->graph_captures;
+  }
+  HIPCHK(c, hipGraphLaunch(c->vbg.exec, c->stream));
+  ++c->graph_replays;
+  return SGDLM_OK;
+}
+
+/* The autotuner would time configurations on this call's draws. */
+bool tuning_pending(sgdlm_ctx *c) {
+  return !lu_pinned(c) && c->autotune &&
+         !(c->tuned_key == structure_key(c) && c->tuned.max_front == c->max_front_used);
+}
+
 /* VB on the context's simulation pool (results stay on the device). */
 int run_vb(sgdlm_ctx *c) {
   int rc;
@@ -804,7 +903,11 @@ int run_vb(sgdlm_ctx *c) {
   D.lu_flags = SIM<int>(c, "lu_flags");
   D.lu_work = c->sim["lu_work"].ptr;
   D.B = vb_buffers(c);
-  if ((rc = run_vb_stages(c, (long)c->nsim, D, plan, true, true)))
+  if (c->graph_mode && !c->timing && !tuning_pending(c))
+    rc = run_vb_graph(c, (long)c->nsim, D, plan);
+  else
+    rc = run_vb_stages(c, (long)c->nsim, D, plan, true, true);
+  if (rc)
     return rc;
   c->lu_flags_valid = true;
   c->vb_valid = true;
@@ -881,6 +984,7 @@ int sgdlm_destroy(sgdlm_ctx *c) {
   if (!c)
     return SGDLM_OK;
   (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   free_pool(c->sim);
   free_pool(c->mem);
   for (auto &pe : c->pending) {
@@ -918,8 +1022,10 @@ int sgdlm_clear_memory(sgdlm_ctx *c) {
   if (!c)
     return SGDLM_ERR_ARG;
   (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   free_pool(c->sim);
   free_pool(c->mem);
+  ++c->gen;
   c->memory_initialized = c->sim_memory_initialized = c->evo_memory_initialized = false;
   c->parents_set = false;
   c->vb_valid = false;
@@ -1038,7 +1144,9 @@ int sgdlm_clear_sim_memory(sgdlm_ctx *c) {
   if (rc)
     return rc;
   (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   free_pool(c->sim);
+  ++c->gen;
   c->sim_memory_initialized = false;
   c->vb_valid = false;
   c->lu_flags_valid = false;
@@ -1640,6 +1748,27 @@ int sgdlm_set_lu_kernel(sgdlm_ctx *c, int kind) {
   if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_PIVOTING)
     return fail(c, SGDLM_ERR_ARG, "unknown LU kernel kind %d", kind);
   c->lu_kind = kind;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_graph_mode(sgdlm_ctx *c, int enable) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  c->graph_mode = enable != 0;
+  if (!c->graph_mode) {
+    (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+  }
+  return SGDLM_OK;
+}
+
+int sgdlm_get_graph_stats(const sgdlm_ctx *c, long *captures, long *replays) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (captures)
+    *captures = c->graph_captures;
+  if (replays)
+    *replays = c->graph_replays;
   return SGDLM_OK;
 }
 

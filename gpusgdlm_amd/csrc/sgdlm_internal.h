@@ -61,10 +61,12 @@ void launch_prior(const Structure &S, double *m_t, double *C_t, double *n_t,
 /* ---- draws (k_draws.hip) ---- */
 void launch_cholesky(const Structure &S, const double *C, double *U,
                      hipStream_t st);
+/* epoch_dev (optional): the epoch is read from this device word instead
+ * of the argument (a captured graph replayed at successive epochs). */
 void launch_draw(const Structure &S, long N, const double *m_t,
                  const double *U, const double *n_t, const double *s_t,
                  uint64_t seed, uint32_t epoch, double *lam, double *theta,
-                 hipStream_t st);
+                 hipStream_t st, const uint32_t *epoch_dev = nullptr);
 void launch_forecast_rhs(const Structure &S, long N, const double *x,
                          const double *m_t, const double *theta,
                          const double *lam, uint64_t seed, uint32_t epoch,
@@ -121,6 +123,9 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             int want_threads, int want_nb, LuPlan *plan, int want_kind = SGDLM_LU_AUTO);
 /* SGDLM_LU_* kind of the kernel that factors the dense block under plan. */
 int lu_plan_kind(const LuPlan &plan);
+/* *p = v on stream st by a one-thread kernel (capturable in a graph,
+ * unlike a memset node). */
+int set_device_int(int *p, int v, hipStream_t st);
 /* Shrinks the persistent grids of a plan (the kernels loop over draws, so
  * fewer workgroups change only the speed) until its workspace fits in
  * work_cap bytes; SGDLM_ERR_ARG if not even one workgroup fits. */

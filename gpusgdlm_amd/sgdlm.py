@@ -391,6 +391,17 @@ class SGDLM:
         inv = {v: k for k, v in self.LU_KINDS.items()}
         self._call("sgdlm_set_lu_kernel", C.c_int(inv[kind]))
 
+    def set_graph_mode(self, on: bool = True):
+        """Capture the device part of a VB call as a hipGraph once and replay
+        it on later calls (bit-identical results)."""
+        self._call("sgdlm_set_graph_mode", C.c_int(1 if on else 0))
+
+    def graph_stats(self):
+        """VB graphs captured and VB calls replayed from one."""
+        cap, rep = C.c_long(), C.c_long()
+        self._call("sgdlm_get_graph_stats", C.byref(cap), C.byref(rep))
+        return {"captures": cap.value, "replays": rep.value}
+
     def set_lu_autotune(self, on: bool = True):
         """Time the feasible VB LU configurations on the first VB call of a
         structure and keep the fastest (default on)."""

@@ -258,6 +258,21 @@ int sgdlm_set_lu_kernel(sgdlm_ctx *ctx, int kind);
  * every new context by setting the environment variable
  * SGDLM_LU_AUTOTUNE=0).  No counterpart in the reference. */
 int sgdlm_set_lu_autotune(sgdlm_ctx *ctx, int enable);
+/* Graph mode (default off): the device part of each VB call (Cholesky,
+ * draws, LU log-determinants, weights, moments and moment-match, the copy
+ * of the weight statistics) is captured once as a hipGraph and replayed on
+ * later calls; the Philox epoch reaches the draw kernels through a device
+ * word that a one-thread kernel sets before each replay, and the LU's
+ * fallback counter is cleared by a kernel, so no node depends on the call.
+ * A call re-captures when anything the graph baked in changed (draw count,
+ * buffers, precision, seed, parental structure, LU plan); calls with stage
+ * timing on, or whose LU configuration is still to be tuned, run the
+ * stages directly.  Results are bit-identical to the direct path.  No
+ * counterpart in the reference (SGDLM.cu:232-503 launches each stage). */
+int sgdlm_set_graph_mode(sgdlm_ctx *ctx, int enable);
+/* How many VB graphs this context captured and how many VB calls it ran
+ * by replaying one. */
+int sgdlm_get_graph_stats(const sgdlm_ctx *ctx, long *captures, long *replays);
 /* The tuned configuration for the current structure (kind = SGDLM_LU_AUTO
  * when none is), the milliseconds the tuning took in this process (0 when
  * taken from the table), the number of configurations timed and whether it
