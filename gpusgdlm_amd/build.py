@@ -25,8 +25,8 @@ LIB = os.path.join(HERE, "lib")
 OBJ = os.path.join(LIB, "obj")
 
 ARCH = "gfx950"
-HIP_SOURCES = ["k_filter.hip", "k_draws.hip", "k_lu.hip", "k_lu_rs.hip", "k_vb.hip", "sgdlm_capi.hip",
-               "lu_symbolic.cpp"]
+HIP_SOURCES = ["k_filter.hip", "k_draws.hip", "k_lu.hip", "k_lu_rs.hip", "k_lu_wave.hip", "k_vb.hip",
+               "sgdlm_capi.hip", "lu_symbolic.cpp"]
 HOST_SOURCES = ["host_wrapper.cpp"]
 
 HIPCC_FLAGS = [

@@ -1655,6 +1655,16 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   }
   if (want_kind == SGDLM_LU_REGISTER && !plan->rs_MT)
     return SGDLM_ERR_ARG;
+  /* small models: one wavefront per draw on the whole matrix */
+  plan->wave_M = 0;
+  plan->wave_grid = 0;
+  if (want_kind == SGDLM_LU_WAVE) {
+    if (solve || !(plan->wave_M = lu_wave_dim(S.M, prec)))
+      return SGDLM_ERR_ARG;
+    plan->wave_grid = lu_wave_grid(plan->wave_M, prec, n_cu);
+    plan->nb = 0;
+    plan->rs_MT = 0;
+  }
   plan->work_bytes =
       (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * M * M * ts;
   if (plan->slot_stride) {
@@ -1667,6 +1677,8 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
 int set_device_int(int *p, int v, hipStream_t st) { return set_int(p, v, st); }
 
 int lu_plan_kind(const LuPlan &plan) {
+  if (plan.wave_M > 0)
+    return SGDLM_LU_WAVE;
   if (plan.rs_MT > 0)
     return SGDLM_LU_REGISTER;
   return plan.nb ? SGDLM_LU_BLOCKED : SGDLM_LU_PIVOTING;
@@ -1702,6 +1714,12 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   a.logdet = logdet;
   a.flag_count = flags;
   a.flag_list = flags + 1;
+  if (plan.wave_M > 0) {
+    /* every draw pivoted by its wavefront: no fallback draws */
+    if (set_int(flags, 0, st))
+      return SGDLM_ERR_DEVICE;
+    return launch_lu_wave(S, plan, N, theta, logdet, st);
+  }
   if (plan.nb == 0 && plan.rs_MT == 0) {
     /* no fast configuration: partial pivoting for every draw (all of them
      * count as fallback draws) */

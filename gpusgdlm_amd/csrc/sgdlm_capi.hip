@@ -662,6 +662,9 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
   /* always feasible: row interchanges for every draw (the only kind for
    * dense blocks beyond the fast kernels' LDS) */
   cands.push_back({SGDLM_LU_PIVOTING, 0, 0});
+  /* small models: one wavefront per draw (no prefix: timed at the default
+   * front only) */
+  const bool wave_ok = lu_wave_dim(c->S.M, c->prec) > 0;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -673,7 +676,10 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
   for (int mf : fronts) {
     if (mf != c->max_front_used && (rc = build_elimination(c, c->ents, mf)))
       break;
-    for (const Cand &k : cands) {
+    std::vector<Cand> cs = cands;
+    if (wave_ok && mf == c->max_front_default)
+      cs.push_back({SGDLM_LU_WAVE, 0, 0});
+    for (const Cand &k : cs) {
       LuPlan p;
       if (lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, k.threads, k.nb, &p, k.kind))
         continue;
@@ -834,7 +840,7 @@ std::vector<uintptr_t> vb_signature(sgdlm_ctx *c, long N, const VbDev &D, const 
           (uintptr_t)p.threads, (uintptr_t)p.fast_grid, (uintptr_t)p.lds, (uintptr_t)p.grid,
           (uintptr_t)p.rs_MT, (uintptr_t)p.rs_grid, (uintptr_t)p.rs_lds, (uintptr_t)p.pivot_nb,
           (uintptr_t)p.pivot_grid, (uintptr_t)p.pivot_lds, (uintptr_t)p.pivot_schur_M,
-          (uintptr_t)p.work_bytes};
+          (uintptr_t)p.work_bytes, (uintptr_t)p.wave_M, (uintptr_t)p.wave_grid};
 }
 
 /* Graph mode: the VB step of run_vb captured once as a hipGraph (re-captured
@@ -1711,15 +1717,15 @@ int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, in
   int rc = vb_lu_plan(c, &plan);
   if (rc)
     return fail(c, rc, "no LU configuration for threads=%d nb=%d", c->lu_threads, c->lu_nb);
-  const bool reg = plan.rs_MT > 0;
+  const bool reg = plan.rs_MT > 0, wave = plan.wave_M > 0;
   if (grid)
-    *grid = reg ? plan.rs_grid : (plan.nb ? plan.fast_grid : plan.grid);
+    *grid = wave ? plan.wave_grid : reg ? plan.rs_grid : (plan.nb ? plan.fast_grid : plan.grid);
   if (lds)
-    *lds = reg ? plan.rs_lds : plan.lds;
+    *lds = wave ? 0 : reg ? plan.rs_lds : plan.lds;
   if (nb)
-    *nb = reg ? 16 : plan.nb;
+    *nb = wave ? 0 : reg ? 16 : plan.nb;
   if (threads)
-    *threads = reg ? 512 : (plan.nb ? plan.threads : 0);
+    *threads = wave ? 256 : reg ? 512 : (plan.nb ? plan.threads : 0);
   return SGDLM_OK;
 }
 
@@ -1736,16 +1742,19 @@ int sgdlm_get_lu_kernel(const sgdlm_ctx *cc, int *kind, int *dense_dim, int *piv
   if (kind)
     *kind = lu_plan_kind(plan);
   if (dense_dim)
-    *dense_dim = plan.rs_MT > 0 ? 16 * plan.rs_MT : (plan.nb ? plan.fast_M : c->S.M);
+    *dense_dim = plan.wave_M > 0 ? plan.wave_M
+                 : plan.rs_MT > 0 ? 16 * plan.rs_MT
+                                  : (plan.nb ? plan.fast_M : c->S.M);
   if (pivot_dim)
-    *pivot_dim = plan.pivot_schur_M > 0 ? plan.pivot_schur_M : c->S.M;
+    *pivot_dim = plan.wave_M > 0 ? plan.wave_M
+                 : plan.pivot_schur_M > 0 ? plan.pivot_schur_M : c->S.M;
   return SGDLM_OK;
 }
 
 int sgdlm_set_lu_kernel(sgdlm_ctx *c, int kind) {
   if (!c)
     return SGDLM_ERR_ARG;
-  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_PIVOTING)
+  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_WAVE)
     return fail(c, SGDLM_ERR_ARG, "unknown LU kernel kind %d", kind);
   c->lu_kind = kind;
   return SGDLM_OK;

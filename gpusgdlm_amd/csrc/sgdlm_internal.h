@@ -106,6 +106,9 @@ struct LuPlan {
    * interchanges (0: the whole matrix); LDS offset of its prefix flag */
   int pivot_schur_M = 0;
   size_t pivot_flag_off = 0;
+  /* small models: one wavefront per draw over the whole matrix in
+   * registers, wave_M rows (k_lu_wave.hip); 0 = not used */
+  int wave_M = 0, wave_grid = 0;
   size_t work_bytes = 0;      /* global workspace bytes                      */
   /* register-resident LU of the dense block (k_lu_rs.hip) instead of the
    * fast kernel: rs_MT > 0 */
@@ -123,6 +126,13 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
             int want_threads, int want_nb, LuPlan *plan, int want_kind = SGDLM_LU_AUTO);
 /* SGDLM_LU_* kind of the kernel that factors the dense block under plan. */
 int lu_plan_kind(const LuPlan &plan);
+/* One-wavefront-per-draw LU (k_lu_wave.hip): its matrix dimension for a
+ * model of M rows (m rounded up to 16) at precision prec (0: too large),
+ * the workgroups that stay resident on n_cu CUs, and the launch. */
+int lu_wave_dim(int M, int prec);
+int lu_wave_grid(int wave_M, int prec, int n_cu);
+int launch_lu_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                   double *logdet, hipStream_t st);
 /* *p = v on stream st by a one-thread kernel (capturable in a graph,
  * unlike a memset node). */
 int set_device_int(int *p, int v, hipStream_t st);

@@ -364,7 +364,7 @@ class SGDLM:
         self._call("sgdlm_get_lu_fallback_count", C.byref(n))
         return n.value
 
-    LU_KINDS = {0: "auto", 1: "blocked", 2: "register", 3: "pivoting"}
+    LU_KINDS = {0: "auto", 1: "blocked", 2: "register", 3: "pivoting", 4: "wave"}
 
     def lu_config(self):
         """Launch shape of the VB LU kernel for the current structure and
@@ -387,7 +387,8 @@ class SGDLM:
         return {"kernel": self.LU_KINDS[k.value], "dense_dim": dd.value, "pivot_dim": pd.value}
 
     def set_lu_kernel(self, kind: str = "auto"):
-        """Pin the VB LU kernel kind ("auto", "blocked", "register", "pivoting")."""
+        """Pin the VB LU kernel kind ("auto", "blocked", "register", "pivoting",
+        "wave")."""
         inv = {v: k for k, v in self.LU_KINDS.items()}
         self._call("sgdlm_set_lu_kernel", C.c_int(inv[kind]))
 

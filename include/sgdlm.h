@@ -52,6 +52,9 @@ extern "C" {
 #define SGDLM_LU_REGISTER 2 /* dense block resident in one CU's registers
                                (FP32 / BF16x3, up to 288 rows)              */
 #define SGDLM_LU_PIVOTING 3 /* every draw with row interchanges             */
+#define SGDLM_LU_WAVE 4     /* small models (m <= 128 FP32, <= 64 FP64): one
+                               wavefront per draw, the whole matrix in
+                               registers, partial pivoting                  */
 
 typedef struct sgdlm_ctx sgdlm_ctx;
 
@@ -232,11 +235,12 @@ int sgdlm_get_lu_elimination(const sgdlm_ctx *ctx, int *max_front,
                              long *prefix_fma);
 /* The kernel that factors the dense block of the VB log-determinant LU
  * for the current structure and precision (SGDLM_LU_BLOCKED / _REGISTER /
- * _PIVOTING), the dimension it factors (the Schur complement after the
- * sparse prefix, rounded up to 16) and the dimension the partial-pivoting
- * kernel factors for a draw that needs row interchanges (the Schur
- * complement when the prefix is re-eliminated, else the whole matrix
- * rounded up to 16).  No counterpart in the reference, where cuBLAS picks
+ * _PIVOTING / _WAVE), the dimension it factors (the Schur complement after
+ * the sparse prefix rounded up to 16; for _WAVE the whole matrix rounded up
+ * to 32, 64 or 128) and the dimension the partial-pivoting kernel factors
+ * for a draw that needs row interchanges (the Schur complement when the
+ * prefix is re-eliminated, else the whole matrix rounded up to 16; _WAVE
+ * pivots every draw itself).  No counterpart in the reference, where cuBLAS picks
  * its getrfBatched kernel (rSGDLM/src/SGDLM/SGDLM.cu:280-281). */
 int sgdlm_get_lu_kernel(const sgdlm_ctx *ctx, int *kind, int *dense_dim,
                         int *pivot_dim);
@@ -246,7 +250,8 @@ int sgdlm_get_lu_kernel(const sgdlm_ctx *ctx, int *kind, int *dense_dim,
 int sgdlm_set_lu_kernel(sgdlm_ctx *ctx, int kind);
 /* Autotuner of the VB LU (default on): on the first VB call for a parental
  * structure and precision, every feasible configuration (kernel kind,
- * workgroup size, panel width, and prefix front limits 24 / default / 48)
+ * workgroup size, panel width, one wavefront per draw for small models,
+ * and prefix front limits 24 / default / 48)
  * is timed on up to 4096 of that call's own draws, and the fastest is kept
  * for the structure, in a table shared by all contexts of the process
  * (keyed by a hash of m, max_p, the parental sets, precision and device).
