@@ -195,6 +195,9 @@ def test_logdet_pivoting_fallback(hiplib, prec, tol):
     the oracle's LU."""
     mod = readme_model(T=30)
     g, o = make_pair(mod, prec)
+    # the no-interchange kernel and its fallback (the autotuner would pick
+    # the one-wavefront kernel for m = 6, which pivots every draw itself)
+    g.set_lu_kernel("blocked")
     filter_steps(g, mod, 1, 10)
     filter_steps(o, mod, 1, 10)
     g.computePosterior(mod.y[10], mod.F(10))
