@@ -107,8 +107,10 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
     """Many independent models (K instances, m log-uniform on [20, 400],
     k_j ~ U{0..min(20, m-1)}, T_filter naive filter steps each, N draws,
     FP32 LU): VB steps of every instance per second through one batched
-    call per step, vs one call per instance; per-instance step times
-    (one call each, synchronous) as p50 / p99."""
+    call per step (one launch per stage over all models, replayed from a
+    captured hipGraph; also eagerly with stage times, and with each model
+    on its own stream), vs one call per instance; batch step times and
+    per-instance step times (one call each, synchronous) as p50 / p99."""
     from gpusgdlm_amd import SGDLM, batch_vb
     from gpusgdlm_amd.workloads import configure, filter_steps, instance_models
     mods = instance_models(K, T=T_filter + 2, seed=seed)
@@ -127,21 +129,32 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
     kinds = {}
     for t in tunes:
         kinds[t["kernel"]] = kinds.get(t["kernel"], 0) + 1
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch_vb(gs, N)
-    tb = time.perf_counter() - t0
-    # graph mode: each context's VB step captured once, then replayed
-    for g in gs:
-        g.set_graph_mode(True)
+    def timed_batches(n):
+        out = []
+        for _ in range(n):
+            t1 = time.perf_counter()
+            batch_vb(gs, N)
+            out.append(time.perf_counter() - t1)
+        return out
+    # one launch per stage over all models (the default batch mode)
     batch_vb(gs, N)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch_vb(gs, N)
-    tg = time.perf_counter() - t0
-    replays = sum(g.graph_stats()["replays"] for g in gs)
-    for g in gs:
-        g.set_graph_mode(False)
+    tb = timed_batches(steps)
+    lead = gs[0]
+    replays = lead.graph_stats()["replays"]
+    # the same stages run eagerly (stage timing on), with HIP-event stage times
+    lead.set_timing(True)
+    t1 = time.perf_counter()
+    batch_vb(gs, N)
+    eager_ms = 1e3 * (time.perf_counter() - t1)
+    stages = {k: round(lead.get_timing(k)[0], 3) for k in
+              ("batch_draw", "batch_lu", "batch_weights", "batch_moments")}
+    lead.set_timing(False)
+    # each model's stages on its own stream, enqueued before any wait
+    lead.set_batch_mode("streams")
+    batch_vb(gs, N)
+    tsm = timed_batches(steps)
+    lead.set_batch_mode("group")
+    bstats = lead.batch_stats()
     for g in gs:
         g.vb_device(N, N)
     per = []
@@ -157,11 +170,17 @@ def run_instances(K, N, steps, T_filter=250, seed=20250102):
     ms = [md.m for md in mods]
     return {"instances": K, "nsim": N, "m_distribution": "log-uniform on [20, 400]",
             "m_quantiles": [int(np.percentile(ms, q)) for q in (0, 50, 90, 100)], "steps": steps,
-            "batched_instance_vb_steps_per_s": K * steps / tb,
+            "batched_instance_vb_steps_per_s": K * steps / sum(tb),
+            "batched_ms_per_step": 1e3 * sum(tb) / steps,
+            "batched_step_ms_p50": 1e3 * float(np.percentile(tb, 50)),
+            "batched_step_ms_p99": 1e3 * float(np.percentile(tb, 99)),
+            "batched_graph_replays": replays,
+            "batched_eager_ms_per_step": eager_ms,
+            "batched_eager_stage_ms": stages,
+            "streams_instance_vb_steps_per_s": K * steps / sum(tsm),
             "sequential_instance_vb_steps_per_s": K * steps / ts,
-            "batched_ms_per_step": 1e3 * tb / steps,
-            "batched_graph_instance_vb_steps_per_s": K * steps / tg,
-            "graph_replays": replays,
+            "batched_vs_sequential": ts / sum(tb),
+            "batch_calls": bstats,
             "instance_step_ms_p50": 1e3 * float(np.percentile(per, 50)),
             "instance_step_ms_p99": 1e3 * float(np.percentile(per, 99)),
             "first_batched_call_s": first_s,
@@ -523,7 +542,7 @@ def main():
                                                 0 if a.no_errors else 1000)
     if a.instances:
         log(f"[instances] {a.instances} models, N={a.instances_nsim}")
-        line["many_instances"] = run_instances(a.instances, a.instances_nsim, 3)
+        line["many_instances"] = run_instances(a.instances, a.instances_nsim, 5)
         log(f"[instances] {line['many_instances']}")
     if not a.no_cpu:
         cb = cpu_baseline(mod, a.filter_steps, N)

@@ -23,6 +23,9 @@ PREC_FP32 = 1
 PREC_BF16X3 = 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
+BATCH_GROUP = 0
+BATCH_STREAMS = 1
+
 _dp = C.POINTER(C.c_double)
 _up = C.POINTER(C.c_uint)
 _ctx = C.c_void_p
@@ -82,6 +85,8 @@ SIGNATURES = {
                                         C.POINTER(C.c_int), _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sgdlm_set_lu_max_front": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_batch_compute_vb_posterior": (C.c_int, [C.POINTER(_ctx), C.c_size_t, C.POINTER(C.c_int)]),
+    "sgdlm_set_batch_mode": (C.c_int, [_ctx, C.c_int]),
+    "sgdlm_get_batch_stats": (C.c_int, [_ctx, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "sgdlm_get_lu_elimination": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.POINTER(C.c_long)]),
 }

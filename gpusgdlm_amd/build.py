@@ -55,7 +55,19 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write(out + "\n")
 
 
-def build(jobs: int = 4, verbose: bool = True) -> str:
+def _stale(src: str, obj: str) -> bool:
+    """obj older than src or than any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    heads.append(os.path.join(ROOT, "include", "sgdlm.h"))
+    return any(os.path.getmtime(f) > t for f in [src] + heads)
+
+
+def build(jobs: int = 4, verbose: bool = True, incremental: bool = False) -> str:
+    """Compiles every source (incremental: only those older than their
+    object file or a header) and links the libraries."""
     os.makedirs(OBJ, exist_ok=True)
     hipcc = _hipcc()
     objs = []
@@ -63,6 +75,8 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     for src in HIP_SOURCES:
         o = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         objs.append(o)
+        if incremental and not _stale(os.path.join(CSRC, src), o):
+            continue
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, cmds))
@@ -101,5 +115,7 @@ def build_oracle() -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--incremental", action="store_true",
+                    help="compile only sources older than their object or a header")
     a = ap.parse_args()
-    build(a.jobs)
+    build(a.jobs, incremental=a.incremental)

@@ -41,10 +41,10 @@ dim3 grid_nm(long N, int m) {
  * sg_cholesky_upper with one element per lane and a barrier per column.
  * Every element sees the same operations on the same operands in the same
  * order as the serial loop, so U is bit-identical to it. */
-__global__ __launch_bounds__(64) void k_cholesky(int m, int max_p, const unsigned *p,
-                                                 const double *C, double *U) {
+__device__ __forceinline__ void cholesky_body(int j, int max_p, const unsigned *p,
+                                              const double *C, double *U) {
   extern __shared__ double a[]; /* [max_p][max_p], a[k * max_p + n] */
-  const int j = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int pj = (int)p[j], ld = max_p, nn = max_p * max_p;
   const size_t o = (size_t)j * nn;
   for (int e = t; e < nn; e += 64) {
@@ -81,6 +81,20 @@ __global__ __launch_bounds__(64) void k_cholesky(int m, int max_p, const unsigne
     U[o + e] = a[e];
 }
 
+__global__ __launch_bounds__(64) void k_cholesky(int m, int max_p, const unsigned *p,
+                                                 const double *C, double *U) {
+  (void)m;
+  cholesky_body(blockIdx.x, max_p, p, C, U);
+}
+
+/* k_cholesky over the series of several models (one workgroup per series
+ * of every model; dynamic LDS for the largest max_p). */
+__global__ __launch_bounds__(64) void k_cholesky_grp(const VbModel *__restrict__ tab,
+                                                     const int *__restrict__ ser_model) {
+  const VbModel &md = tab[ser_model[blockIdx.x]];
+  cholesky_body((int)blockIdx.x - md.ser0, md.max_p, md.p, md.C, md.U);
+}
+
 __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
                                              const unsigned *p, const int *off,
                                              long P, const double *m_t,
@@ -108,19 +122,16 @@ __global__ __launch_bounds__(TB) void k_draw(long N, int m, int max_p,
  * that bounds k_draw's occupancy.  p_j is uniform over a workgroup, so the
  * unrolled loops' bounds checks are uniform branches. */
 template <int PMAX>
-__global__ __launch_bounds__(TB) void k_draw_reg(long N, int m, int max_p,
-                                                 const unsigned *p, const int *off,
-                                                 long P, const double *m_t,
-                                                 const double *U, const double *n_t,
-                                                 const double *s_t, uint64_t seed,
-                                                 uint32_t epoch_arg, const uint32_t *epoch_dev,
-                                                 double *lam, double *theta) {
-  const uint32_t epoch = epoch_dev ? *epoch_dev : epoch_arg;
-  const int j = blockIdx.y;
+__device__ __forceinline__ void draw_reg_body(long N, int m, int max_p, const unsigned *p,
+                                              const int *off, long P, const double *m_t,
+                                              const double *U, const double *n_t,
+                                              const double *s_t, uint64_t seed, uint32_t epoch,
+                                              double *lam, double *theta, int j, long bx,
+                                              long gx) {
   const int pj = (int)p[j];
   const double *Uj = U + (size_t)j * max_p * max_p, *mj = m_t + (size_t)j * max_p;
   const double nj = n_t[j], sj = s_t[j];
-  for (long i = (long)blockIdx.x * TB + threadIdx.x; i < N; i += (long)gridDim.x * TB) {
+  for (long i = bx * TB + threadIdx.x; i < N; i += gx * TB) {
     const double l = sg_gamma(0.5 * nj, 0.5 * nj * sj, seed, epoch, SG_PURPOSE_LAMBDA,
                               (uint32_t)i, (uint32_t)j);
     const double scale = __builtin_sqrt(sj * l);
@@ -153,6 +164,35 @@ __global__ __launch_bounds__(TB) void k_draw_reg(long N, int m, int max_p,
       if (k < pj)
         out[k] = mj[k] + acc[k] / scale;
   }
+}
+
+template <int PMAX>
+__global__ __launch_bounds__(TB) void k_draw_reg(long N, int m, int max_p,
+                                                 const unsigned *p, const int *off,
+                                                 long P, const double *m_t,
+                                                 const double *U, const double *n_t,
+                                                 const double *s_t, uint64_t seed,
+                                                 uint32_t epoch_arg, const uint32_t *epoch_dev,
+                                                 double *lam, double *theta) {
+  const uint32_t epoch = epoch_dev ? *epoch_dev : epoch_arg;
+  draw_reg_body<PMAX>(N, m, max_p, p, off, P, m_t, U, n_t, s_t, seed, epoch, lam, theta,
+                      (int)blockIdx.y, (long)blockIdx.x, (long)gridDim.x);
+}
+
+/* k_draw_reg over the series of several models: blockIdx.y = series of
+ * all models (ser_model: its model), blockIdx.x = blocks of TB draws
+ * (models with fewer draws leave the high blocks idle); the epoch of model
+ * i is epochs[i].  PMAX covers the largest max_p; the draws are
+ * bit-identical to the single-model launch (same operations per value). */
+template <int PMAX>
+__global__ __launch_bounds__(TB) void k_draw_reg_grp(const VbModel *__restrict__ tab,
+                                                     const int *__restrict__ ser_model,
+                                                     const uint32_t *__restrict__ epochs) {
+  const int mi = ser_model[blockIdx.y];
+  const VbModel &md = tab[mi];
+  draw_reg_body<PMAX>(md.N, md.m, md.max_p, md.p, md.off, md.P, md.m_t, md.U, md.n_t, md.s_t,
+                      md.seed, epochs[mi], md.lam, md.theta, (int)blockIdx.y - md.ser0,
+                      (long)blockIdx.x, (long)gridDim.x);
 }
 
 /* rhs_ij = nu_ij / sqrt(lambda_ij) + sum_{k: sp_jk external} x_jk theta_ijk
@@ -373,6 +413,22 @@ void launch_evolve(const Structure &S, long N, const double *U_W,
   hipLaunchKernelGGL(k_evolve, grid_nm(N, S.m), dim3(TB), lds, st, N, S.m,
                      S.max_p, S.p, S.off, S.P, U_W, r_t, c_t, beta, G, m_t,
                      seed, epoch, lam, theta);
+}
+
+void launch_cholesky_grp(const VbModel *tab, const int *ser_model, int n_series, int max_p,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_cholesky_grp, dim3(n_series), dim3(64),
+                     (size_t)max_p * max_p * sizeof(double), st, tab, ser_model);
+}
+
+int launch_draw_grp(const VbModel *tab, const int *ser_model, const uint32_t *epochs,
+                    int n_series, long max_N, int max_p, hipStream_t st) {
+  if (max_p > 32 || n_series > 65535)
+    return SGDLM_ERR_ARG;
+  auto kern = max_p <= 8 ? k_draw_reg_grp<8> : max_p <= 16 ? k_draw_reg_grp<16>
+              : max_p <= 24 ? k_draw_reg_grp<24> : k_draw_reg_grp<32>;
+  hipLaunchKernelGGL(kern, grid_nm(max_N, n_series), dim3(TB), 0, st, tab, ser_model, epochs);
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 
 void launch_unpack_theta(const Structure &S, long N, const double *theta,

@@ -165,8 +165,7 @@ void launch_prior(const Structure &S, double *m_t, double *C_t, double *n_t,
                   double *W_t, const double *beta, const double *delta,
                   const double *G, hipStream_t st) {
   const size_t lds = (2 * (size_t)S.max_p * S.max_p + S.max_p) * sizeof(double);
-  (void)hipFuncSetAttribute((const void *)k_prior,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  (void)allow_dynamic_lds((const void *)k_prior, lds);
   hipLaunchKernelGGL(k_prior, dim3(S.m), dim3(WAVE), lds, st, S.m, S.max_p,
                      S.p, m_t, C_t, n_t, W_t, beta, delta, G);
 }

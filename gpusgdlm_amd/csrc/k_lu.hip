@@ -57,7 +57,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
+#include <map>
 #include <math.h>
+#include <mutex>
 
 #include "lu_prefix.h"
 #include "lu_symbolic.h"
@@ -532,8 +535,9 @@ __device__ void diag_block_inverses(const T *Pn, T *Linv, int nq, int wave,
 /* ========================================================================
  * Fast path: no row interchanges, verified afterwards.
  * ====================================================================== */
-template <typename T, int NB, int MODE, int RPT, int NT, int NBI, bool SOLVE>
-__global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
+template <typename T, int NB, int MODE, int RPT, int NT, int NBI, bool SOLVE, bool GRP>
+__global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a0, const LuArgs *__restrict__ tab,
+                                                   GrpMap gm) {
   static_assert(NB % NBI == 0 && (MODE == MODE_NATIVE || NBI == NB), "sub-panel width");
   constexpr int NS = NB / NBI; /* register sub-panels per panel */
   typedef typename Vec4<T>::type V4;
@@ -545,6 +549,8 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
   typedef typename Vec16<T>::type VV;
   constexpr int VW = Vec16<T>::n;
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  int bid, nbk;
+  const LuArgs a = grp_args<GRP>(a0, tab, gm, bid, nbk);
 
   const int M = a.M;
   const int tid = threadIdx.x;
@@ -561,9 +567,9 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
   float *redf = reinterpret_cast<float *>(redd + NW);
   auto row_of = [](int i) -> int { return i; };
 
-  T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * M * M;
+  T *A = reinterpret_cast<T *>(a.work) + (size_t)bid * M * M;
 
-  for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
+  for (long d = bid; d < a.n_draws; d += nbk) {
     int bad = 0;
     T amax = (T)0;
     double ls = 0.0;
@@ -575,7 +581,7 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
     if (tid == 0)
       *eflag = 0;
     prefix_assemble<T>(A, M, smem, a.theta + (size_t)d * a.P, a, tid, NT,
-                       !a.e_tab_persist || d == (long)blockIdx.x, bad, amax, ls, eflag,
+                       !a.e_tab_persist || d == (long)bid, bad, amax, ls, eflag,
                        (T)kMaxMultiplier);
     bool stop = *eflag != 0;
     if constexpr (SOLVE) {
@@ -583,7 +589,7 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
         /* the dense LU reuses the prefix slots' LDS: keep them for the
          * substitutions */
         T *save = reinterpret_cast<T *>(reinterpret_cast<char *>(a.e_slot_save) +
-                                        (size_t)blockIdx.x * a.e_slot_stride);
+                                        (size_t)bid * a.e_slot_stride);
         const T *sl = reinterpret_cast<const T *>(smem + a.e_pre_base);
         for (int q = tid; q < a.e_ns; q += NT)
           save[q] = sl[q];
@@ -976,7 +982,7 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
           int *tb = reinterpret_cast<int *>(smem + a.e_pre_base + a.e_slot_bytes);
           T *sl = reinterpret_cast<T *>(smem + a.e_pre_base);
           const T *save = reinterpret_cast<const T *>(reinterpret_cast<char *>(a.e_slot_save) +
-                                                      (size_t)blockIdx.x * a.e_slot_stride);
+                                                      (size_t)bid * a.e_slot_stride);
           for (int w = tid; w < a.e_tab_words; w += NT)
             tb[w] = a.e_tab[w];
           for (int q = tid; q < a.e_ns; q += NT)
@@ -992,14 +998,17 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a) {
 /* ========================================================================
  * Partial-pivoting path (fallback for flagged draws, forecast solves).
  * ====================================================================== */
-template <typename T, int NB, int MODE, bool SOLVE>
-__global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a) {
+template <typename T, int NB, int MODE, bool SOLVE, bool GRP>
+__global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a0, const LuArgs *__restrict__ tab,
+                                                  GrpMap gm) {
   typedef typename Vec4<T>::type V4;
   constexpr int WG = 256;
   constexpr int NW = 4;
   constexpr int PS = panel_stride(NB, MODE_NATIVE);
   constexpr int NQ = NB / 16;
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  int bid, nbk;
+  const LuArgs a = grp_args<GRP>(a0, tab, gm, bid, nbk);
 
   const int Mfull = a.M;
   const int tid = threadIdx.x;
@@ -1021,10 +1030,10 @@ __global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a) {
   off = (off + 15) & ~(size_t)15;
   double *sv = reinterpret_cast<double *>(smem + off); /* SOLVE only */
 
-  T *A = reinterpret_cast<T *>(a.work) + (size_t)blockIdx.x * Mfull * Mfull;
+  T *A = reinterpret_cast<T *>(a.work) + (size_t)bid * Mfull * Mfull;
   const long count = a.flag_list ? (long)*a.flag_count : a.n_draws;
 
-  for (long q = blockIdx.x; q < count; q += gridDim.x) {
+  for (long q = bid; q < count; q += nbk) {
     const long d = a.flag_list ? (long)a.flag_list[q] : q;
     const double *th = a.theta + (size_t)d * a.P;
     /* the dimension factored for this draw: the Schur complement after the
@@ -1327,92 +1336,126 @@ int set_int(int *p, int v, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 
-template <typename Kern>
-int launch_kernel(Kern kern, size_t lds, int grid, int threads, const LuArgs &a, hipStream_t st) {
-  if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds) != hipSuccess)
+/* An LU kernel: its single-model form, its form for a launch over several
+ * models (log-determinants only) and its workgroup size. */
+struct LuKern {
+  const void *one = nullptr, *grp = nullptr;
+  int threads = 0;
+};
+
+template <typename T, int NB, int MODE, int RPT, int NT, int NBI, bool SOLVE>
+LuKern fast_k() {
+  LuKern k;
+  k.one = (const void *)k_lu_fast<T, NB, MODE, RPT, NT, NBI, SOLVE, false>;
+  if constexpr (!SOLVE)
+    k.grp = (const void *)k_lu_fast<T, NB, MODE, RPT, NT, NBI, false, true>;
+  k.threads = NT;
+  return k;
+}
+
+template <typename T, int NB, bool SOLVE> LuKern pivot_k() {
+  LuKern k;
+  k.one = (const void *)k_lu_pivot<T, NB, MODE_NATIVE, SOLVE, false>;
+  if constexpr (!SOLVE)
+    k.grp = (const void *)k_lu_pivot<T, NB, MODE_NATIVE, false, true>;
+  k.threads = 256;
+  return k;
+}
+
+/* One model's launch of kernel k (its single-model form). */
+int launch_one(const LuKern &k, size_t lds, int grid, const LuArgs &a, hipStream_t st) {
+  if (!k.one)
+    return SGDLM_ERR_ARG;
+  if (allow_dynamic_lds(k.one, lds) != hipSuccess)
     return SGDLM_ERR_DEVICE;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, a);
+  LuArgs a0 = a;
+  const LuArgs *tab = nullptr;
+  GrpMap gm;
+  void *args[] = {&a0, &tab, &gm};
+  if (hipLaunchKernel(k.one, dim3(grid), dim3(k.threads), args, lds, st) != hipSuccess)
+    return SGDLM_ERR_DEVICE;
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 
-template <typename T, int NB, int MODE, bool SOLVE>
-int fast_rpt(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
+/* The fast kernel for plan at dense dimension M (empty: none fits). */
+template <typename T, int NB, int MODE, bool SOLVE> LuKern fast_sel(const LuPlan &plan, int M) {
   /* register sub-panels of 32 columns for 64-wide native panels */
   constexpr int NBI = (MODE == MODE_NATIVE && NB > 32) ? 32 : NB;
   if (plan.threads == 256) {
     /* two workgroups per CU; rows per thread up to 2 (M <= 512); solves
      * only with 32-wide panels */
     if constexpr (NBI <= 32 && (!SOLVE || NB == 32 || sizeof(T) == 4)) {
-      if (a.M <= 256)
-        return launch_kernel(k_lu_fast<T, NB, MODE, 1, 256, NBI, SOLVE>, plan.lds, plan.fast_grid, 256,
-                             a, st);
+      if (M <= 256)
+        return fast_k<T, NB, MODE, 1, 256, NBI, SOLVE>();
       if constexpr (!(sizeof(T) == 8 && NB == 16)) {
-        if (a.M <= 512)
-          return launch_kernel(k_lu_fast<T, NB, MODE, 2, 256, NBI, SOLVE>, plan.lds, plan.fast_grid,
-                               256, a, st);
+        if (M <= 512)
+          return fast_k<T, NB, MODE, 2, 256, NBI, SOLVE>();
       }
     }
-    return SGDLM_ERR_ARG;
+    return LuKern();
   }
   if (plan.threads != 512)
-    return SGDLM_ERR_ARG;
-  if (a.M <= 512)
-    return launch_kernel(k_lu_fast<T, NB, MODE, 1, 512, NBI, SOLVE>, plan.lds, plan.fast_grid, 512, a, st);
+    return LuKern();
+  if (M <= 512)
+    return fast_k<T, NB, MODE, 1, 512, NBI, SOLVE>();
   /* FP64 with 16-wide panels and two rows per thread is never planned (see
    * lu_plan); not instantiating it also avoids a register-coalescer crash of
    * the ROCm 7.2 compiler on that instantiation. */
   if constexpr (!(sizeof(T) == 8 && NB == 16)) {
-    if (a.M <= 1024)
-      return launch_kernel(k_lu_fast<T, NB, MODE, 2, 512, NBI, SOLVE>, plan.lds, plan.fast_grid, 512, a,
-                           st);
+    if (M <= 1024)
+      return fast_k<T, NB, MODE, 2, 512, NBI, SOLVE>();
   }
-  return SGDLM_ERR_ARG;
+  return LuKern();
+}
+
+template <bool SOLVE> LuKern select_fast(const LuPlan &plan, int M) {
+  switch (plan.prec) {
+  case SGDLM_PREC_FP64:
+    if (plan.nb == 64)
+      return fast_sel<double, 64, MODE_NATIVE, SOLVE>(plan, M);
+    if (plan.nb == 32)
+      return fast_sel<double, 32, MODE_NATIVE, SOLVE>(plan, M);
+    if constexpr (!SOLVE)
+      return fast_sel<double, 16, MODE_NATIVE, false>(plan, M);
+    return LuKern();
+  case SGDLM_PREC_FP32:
+    if (plan.nb == 64)
+      return fast_sel<float, 64, MODE_NATIVE, SOLVE>(plan, M);
+    if (plan.nb == 32)
+      return fast_sel<float, 32, MODE_NATIVE, SOLVE>(plan, M);
+    if constexpr (!SOLVE)
+      return fast_sel<float, 16, MODE_NATIVE, false>(plan, M);
+    return LuKern();
+  case SGDLM_PREC_BF16X3:
+    if constexpr (!SOLVE) {
+      if (plan.nb == 64)
+        return fast_sel<float, 64, MODE_BF16X3, false>(plan, M);
+      return fast_sel<float, 32, MODE_BF16X3, false>(plan, M);
+    }
+    return LuKern();
+  }
+  return LuKern();
+}
+
+template <bool SOLVE> LuKern select_pivot(const LuPlan &plan) {
+  /* BF16x3 falls back to FP32 partial pivoting */
+  if (plan.prec == SGDLM_PREC_FP64)
+    return plan.pivot_nb == 32 ? pivot_k<double, 32, SOLVE>() : pivot_k<double, 16, SOLVE>();
+  if (plan.pivot_nb == 64)
+    return pivot_k<float, 64, SOLVE>();
+  if (plan.pivot_nb == 32)
+    return pivot_k<float, 32, SOLVE>();
+  return pivot_k<float, 16, SOLVE>();
 }
 
 template <bool SOLVE>
 int dispatch_fast(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
-  switch (plan.prec) {
-  case SGDLM_PREC_FP64:
-    if (plan.nb == 64)
-      return fast_rpt<double, 64, MODE_NATIVE, SOLVE>(a, plan, st);
-    if (plan.nb == 32)
-      return fast_rpt<double, 32, MODE_NATIVE, SOLVE>(a, plan, st);
-    if constexpr (!SOLVE)
-      return fast_rpt<double, 16, MODE_NATIVE, false>(a, plan, st);
-    return SGDLM_ERR_ARG;
-  case SGDLM_PREC_FP32:
-    if (plan.nb == 64)
-      return fast_rpt<float, 64, MODE_NATIVE, SOLVE>(a, plan, st);
-    if (plan.nb == 32)
-      return fast_rpt<float, 32, MODE_NATIVE, SOLVE>(a, plan, st);
-    if constexpr (!SOLVE)
-      return fast_rpt<float, 16, MODE_NATIVE, false>(a, plan, st);
-    return SGDLM_ERR_ARG;
-  case SGDLM_PREC_BF16X3:
-    if constexpr (!SOLVE) {
-      if (plan.nb == 64)
-        return fast_rpt<float, 64, MODE_BF16X3, false>(a, plan, st);
-      return fast_rpt<float, 32, MODE_BF16X3, false>(a, plan, st);
-    }
-    return SGDLM_ERR_ARG;
-  }
-  return SGDLM_ERR_ARG;
+  return launch_one(select_fast<SOLVE>(plan, a.M), plan.lds, plan.fast_grid, a, st);
 }
 
 template <bool SOLVE>
 int dispatch_pivot(const LuArgs &a, const LuPlan &plan, hipStream_t st) {
-  /* BF16x3 falls back to FP32 partial pivoting */
-  if (plan.prec == SGDLM_PREC_FP64) {
-    if (plan.pivot_nb == 32)
-      return launch_kernel(k_lu_pivot<double, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
-    return launch_kernel(k_lu_pivot<double, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
-  }
-  if (plan.pivot_nb == 64)
-    return launch_kernel(k_lu_pivot<float, 64, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
-  if (plan.pivot_nb == 32)
-    return launch_kernel(k_lu_pivot<float, 32, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
-  return launch_kernel(k_lu_pivot<float, 16, MODE_NATIVE, SOLVE>, plan.pivot_lds, plan.pivot_grid, 256, a, st);
+  return launch_one(select_pivot<SOLVE>(plan), plan.pivot_lds, plan.pivot_grid, a, st);
 }
 
 LuArgs base_args(const Structure &S, long N, const double *theta, void *work) {
@@ -1676,6 +1719,30 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
 
 int set_device_int(int *p, int v, hipStream_t st) { return set_int(p, v, st); }
 
+hipError_t allow_dynamic_lds(const void *fn, size_t lds) {
+  static std::mutex mu;
+  static std::map<const void *, size_t> allowed;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = allowed.find(fn);
+  if (it != allowed.end() && it->second >= lds)
+    return hipSuccess;
+  int dev = 0, mx = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    mx = 0;
+  const size_t want = std::max(lds, (size_t)std::max(mx, 0));
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) ==
+      hipSuccess) {
+    allowed[fn] = want;
+    return hipSuccess;
+  }
+  (void)hipGetLastError();
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess)
+    allowed[fn] = lds;
+  return e;
+}
+
 int lu_plan_kind(const LuPlan &plan) {
   if (plan.wave_M > 0)
     return SGDLM_LU_WAVE;
@@ -1769,6 +1836,92 @@ int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
     return rc;
   a.M = S.M;
   return dispatch_pivot<true>(a, plan, st);
+}
+
+int lu_grp_stages_shape(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                        double *logdet, int *flags, int *flag_init, LuGrpKernel *first,
+                        LuGrpKernel *second);
+
+int lu_grp_stages(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                  double *logdet, int *flags, double draw_wg_ms, int *flag_init,
+                  LuGrpKernel *first, LuGrpKernel *second) {
+  *first = LuGrpKernel();
+  *second = LuGrpKernel();
+  const int rc = lu_grp_stages_shape(S, plan, N, theta, logdet, flags, flag_init, first, second);
+  /* the measured time of a draw on one workgroup of the first stage's
+   * kernel replaces the size model */
+  if (!rc && draw_wg_ms > 0.0)
+    first->cost = (double)N * draw_wg_ms;
+  return rc;
+}
+
+int lu_grp_stages_shape(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                        double *logdet, int *flags, int *flag_init, LuGrpKernel *first,
+                        LuGrpKernel *second) {
+  const size_t ts = plan.prec == SGDLM_PREC_FP64 ? 8 : 4;
+  /* the fallback: partial pivoting of the flagged draws (or of every draw) */
+  auto pivot_stage = [&](bool all, LuGrpKernel *k) {
+    LuArgs a = base_args(S, N, theta, nullptr);
+    a.logdet = logdet;
+    a.flag_count = all ? nullptr : flags;
+    a.flag_list = all ? nullptr : flags + 1;
+    set_pivot_layout(a, S, plan);
+    const LuKern lk = select_pivot<false>(plan);
+    if (!lk.grp)
+      return SGDLM_ERR_ARG;
+    k->fn = lk.grp;
+    k->threads = lk.threads;
+    k->per_cu = std::max(1, plan.pivot_grid / std::max(1, plan.grid));
+    k->lds = plan.pivot_lds;
+    k->ws_wg = (size_t)S.M * S.M * ts;
+    k->units = N;
+    const double Md = (double)(plan.pivot_schur_M > 0 ? plan.pivot_schur_M : S.M);
+    /* ms per draw on one workgroup (size model: a latency-bound column
+     * chain plus the cubic updates); every draw, or (fallback) a share of
+     * them that is not known yet */
+    k->cost = (double)N * (1e-3 * Md + 1e-8 * Md * Md * Md) * (all ? 1.0 : 0.05);
+    k->args.assign(reinterpret_cast<const unsigned char *>(&a),
+                   reinterpret_cast<const unsigned char *>(&a) + sizeof(a));
+    k->work_field = (long)offsetof(LuArgs, work);
+    return SGDLM_OK;
+  };
+  if (plan.wave_M > 0) {
+    *flag_init = 0;
+    return lu_grp_wave(S, plan, N, theta, logdet, first);
+  }
+  if (plan.nb == 0 && plan.rs_MT == 0) {
+    *flag_init = (int)N;
+    return pivot_stage(true, first);
+  }
+  *flag_init = 0;
+  int rc;
+  if (plan.rs_MT > 0) {
+    rc = lu_grp_rs(S, plan, N, theta, logdet, flags, first);
+  } else {
+    LuArgs a = base_args(S, N, theta, nullptr);
+    a.logdet = logdet;
+    a.flag_count = flags;
+    a.flag_list = flags + 1;
+    set_fast_layout(a, S, plan);
+    const LuKern lk = select_fast<false>(plan, a.M);
+    rc = lk.grp ? SGDLM_OK : SGDLM_ERR_ARG;
+    if (!rc) {
+      first->fn = lk.grp;
+      first->threads = lk.threads;
+      first->per_cu = std::max(1, plan.fast_grid / std::max(1, plan.grid));
+      first->lds = plan.lds;
+      first->ws_wg = (size_t)plan.fast_M * plan.fast_M * ts;
+      first->units = N;
+      const double Md = (double)plan.fast_M;
+      first->cost = (double)N * (2e-4 * Md + 6e-9 * Md * Md * Md + 5e-7 * (double)S.e_fma);
+      first->args.assign(reinterpret_cast<const unsigned char *>(&a),
+                         reinterpret_cast<const unsigned char *>(&a) + sizeof(a));
+      first->work_field = (long)offsetof(LuArgs, work);
+    }
+  }
+  if (rc)
+    return rc;
+  return pivot_stage(false, second);
 }
 
 } // namespace sgdlm

@@ -215,10 +215,13 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
   }
 }
 
-template <int WR, int WC, int RT, int CT, int MODE, int MTT>
-__global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a) {
+template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool GRP>
+__global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
+    RsArgs a0, const RsArgs *__restrict__ tab, GrpMap gm) {
   constexpr int NW = WR * WC, NT = 64 * NW;
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  int bid, nbk;
+  const RsArgs a = grp_args<GRP>(a0, tab, gm, bid, nbk);
   constexpr unsigned LO = 0xffffu;
   constexpr int UB = 4; /* Schur blocks in flight per wavefront */
   constexpr int RR = 8; /* rounds loaded ahead per block        */
@@ -255,7 +258,7 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a)
   float *Xw = Xb + wr * TSF + 4 * g * TR + l16; /* accumulator layout     */
   constexpr int SST = 16 * WC * CT + 4;        /* staging row stride      */
 
-  for (long d = blockIdx.x; d < a.n_draws; d += gridDim.x) {
+  for (long d = bid; d < a.n_draws; d += nbk) {
     const double *th = a.theta + (size_t)d * a.P;
     int bad = 0;
     float amax = 0.f;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(RsArgs a)
       *flagA = 0;
       *flagB = 0;
     }
-    prefix_eliminate<float>(sl, tb, th, a.pre, tid, NT, !a.tab_persist || d == (long)blockIdx.x, bad,
+    prefix_eliminate<float>(sl, tb, th, a.pre, tid, NT, !a.tab_persist || d == (long)bid, bad,
                             amax, ls, flagA, kMaxMultiplier);
     bool stop = *flagA != 0;
 
@@ -559,8 +562,10 @@ int rs_stage_stride(int MT) {
   return 16 * q.WC * q.CT + 4;
 }
 
-int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
-                 double *logdet, int *flags, hipStream_t st) {
+namespace {
+
+RsArgs rs_args(const Structure &S, const LuPlan &plan, long N, const double *theta,
+               double *logdet, int *flags) {
   RsArgs a{};
   a.MT = plan.rs_MT;
   a.n_draws = N;
@@ -597,23 +602,73 @@ int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *t
   a.sg_idx = S.e_rb_sg_idx;
   a.st_pairs = S.e_rb_pairs;
   a.zero_pair = S.e_zero_pair;
-  void (*kern)(RsArgs) = nullptr;
-  const bool bf = plan.prec == SGDLM_PREC_BF16X3;
-  const RsShape q = rs_shape(a.MT);
-  if (a.MT > kRsMaxTiles)
-    return SGDLM_ERR_ARG;
+  return a;
+}
+
+/* The register kernel for a dense block of MT tiles: single-model and
+ * group forms, workgroup size (one workgroup per CU). */
+struct RsKern {
+  const void *one = nullptr, *grp = nullptr;
+  int threads = 0;
+};
+
+template <int WR, int WC, int RT, int CT, int MODE, int MTT> RsKern rs_k() {
+  RsKern k;
+  k.one = (const void *)k_lu_rs<WR, WC, RT, CT, MODE, MTT, false>;
+  k.grp = (const void *)k_lu_rs<WR, WC, RT, CT, MODE, MTT, true>;
+  k.threads = 64 * WR * WC;
+  return k;
+}
+
+RsKern rs_kern(int MT, bool bf) {
+  const RsShape q = rs_shape(MT);
+  if (MT > kRsMaxTiles || MT < 1)
+    return RsKern();
   if (q.RT == 6)
-    kern = bf ? k_lu_rs<2, 4, 6, 3, RS_BF16X3, 0> : k_lu_rs<2, 4, 6, 3, RS_FP32, 0>;
-  else if (q.RT == 8)
-    kern = bf ? k_lu_rs<2, 4, 8, 4, RS_BF16X3, 0> : k_lu_rs<2, 4, 8, 4, RS_FP32, 0>;
-  else
-    kern = bf ? k_lu_rs<2, 4, 9, 5, RS_BF16X3, 18> : k_lu_rs<2, 4, 9, 5, RS_FP32, 18>;
-  const int threads = 64 * q.WR * q.WC;
-  if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)plan.rs_lds) != hipSuccess)
+    return bf ? rs_k<2, 4, 6, 3, RS_BF16X3, 0>() : rs_k<2, 4, 6, 3, RS_FP32, 0>();
+  if (q.RT == 8)
+    return bf ? rs_k<2, 4, 8, 4, RS_BF16X3, 0>() : rs_k<2, 4, 8, 4, RS_FP32, 0>();
+  return bf ? rs_k<2, 4, 9, 5, RS_BF16X3, 18>() : rs_k<2, 4, 9, 5, RS_FP32, 18>();
+}
+
+} // namespace
+
+int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                 double *logdet, int *flags, hipStream_t st) {
+  RsArgs a = rs_args(S, plan, N, theta, logdet, flags);
+  const RsKern k = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3);
+  if (!k.one)
+    return SGDLM_ERR_ARG;
+  if (allow_dynamic_lds(k.one, plan.rs_lds) != hipSuccess)
     return SGDLM_ERR_DEVICE;
-  hipLaunchKernelGGL(kern, dim3(plan.rs_grid), dim3(threads), plan.rs_lds, st, a);
+  const RsArgs *tab = nullptr;
+  GrpMap gm;
+  void *args[] = {&a, &tab, &gm};
+  if (hipLaunchKernel(k.one, dim3(plan.rs_grid), dim3(k.threads), args, plan.rs_lds, st) !=
+      hipSuccess)
+    return SGDLM_ERR_DEVICE;
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
+int lu_grp_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
+              double *logdet, int *flags, LuGrpKernel *k) {
+  const RsArgs a = rs_args(S, plan, N, theta, logdet, flags);
+  const RsKern rk = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3);
+  if (!rk.grp)
+    return SGDLM_ERR_ARG;
+  k->fn = rk.grp;
+  k->threads = rk.threads;
+  k->per_cu = 1;
+  k->lds = plan.rs_lds;
+  k->ws_wg = 0;
+  k->units = N;
+  /* ms per draw on one workgroup (size model; 0.15 ms at 18 tiles) */
+  const double r = a.MT / 18.0;
+  k->cost = (double)N * (0.15 * r * r + 5e-7 * (double)S.e_fma);
+  k->args.assign(reinterpret_cast<const unsigned char *>(&a),
+                 reinterpret_cast<const unsigned char *>(&a) + sizeof(a));
+  k->work_field = -1;
+  return SGDLM_OK;
 }
 
 } // namespace sgdlm

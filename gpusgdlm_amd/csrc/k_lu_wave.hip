@@ -60,13 +60,16 @@ __device__ __forceinline__ double bcast(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <typename T, int M>
-__global__ __launch_bounds__(256, 1) void k_lu_wave(WaveArgs a) {
+template <typename T, int M, bool GRP>
+__global__ __launch_bounds__(256, 1) void k_lu_wave(WaveArgs a0, const WaveArgs *__restrict__ tab,
+                                                   GrpMap gm) {
   constexpr int R = (M + 63) / 64; /* rows per lane */
   constexpr int G = 8;             /* pivot-row entries read per batch */
   const int lane = threadIdx.x & 63;
-  const long wave0 = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  int bid, nbk;
+  const WaveArgs a = grp_args<GRP>(a0, tab, gm, bid, nbk);
+  const long wave0 = ((long)bid * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)nbk * blockDim.x) >> 6;
   for (long d = wave0; d < a.n_draws; d += nwaves) {
     const double *th = a.theta + (size_t)d * a.P;
     T row[R][M];
@@ -177,13 +180,14 @@ __global__ __launch_bounds__(256, 1) void k_lu_wave(WaveArgs a) {
 }
 
 template <typename T, int M> int launch_wave(const WaveArgs &a, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_lu_wave<T, M>), dim3(grid), dim3(256), 0, st, a);
+  const WaveArgs *tab = nullptr;
+  hipLaunchKernelGGL((k_lu_wave<T, M, false>), dim3(grid), dim3(256), 0, st, a, tab, GrpMap());
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 
 template <typename T, int M> int resident_blocks() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_lu_wave<T, M>, 256, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_lu_wave<T, M, false>, 256, 0) !=
           hipSuccess ||
       n < 1) {
     (void)hipGetLastError();
@@ -216,8 +220,8 @@ int lu_wave_grid(int wave_M, int prec, int n_cu) {
   return std::max(1, n_cu) * per_cu;
 }
 
-int launch_lu_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
-                   double *logdet, hipStream_t st) {
+namespace {
+WaveArgs wave_args(const Structure &S, long N, const double *theta, double *logdet) {
   WaveArgs a;
   a.n_draws = N;
   a.m = S.m;
@@ -227,6 +231,46 @@ int launch_lu_wave(const Structure &S, const LuPlan &plan, long N, const double 
   a.g_col = S.g_col;
   a.g_idx = S.g_idx;
   a.logdet = logdet;
+  return a;
+}
+} // namespace
+
+int lu_grp_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                double *logdet, LuGrpKernel *k) {
+  const WaveArgs a = wave_args(S, N, theta, logdet);
+  const bool f64 = plan.prec == SGDLM_PREC_FP64;
+  k->fn = nullptr;
+  switch (plan.wave_M) {
+  case 32:
+    k->fn = f64 ? (const void *)k_lu_wave<double, 32, true> : (const void *)k_lu_wave<float, 32, true>;
+    break;
+  case 64:
+    k->fn = f64 ? (const void *)k_lu_wave<double, 64, true> : (const void *)k_lu_wave<float, 64, true>;
+    break;
+  case 128:
+    if (!f64)
+      k->fn = (const void *)k_lu_wave<float, 128, true>;
+    break;
+  }
+  if (!k->fn)
+    return SGDLM_ERR_ARG;
+  k->threads = 256;
+  k->per_cu = std::max(1, plan.wave_grid / std::max(1, plan.grid));
+  k->lds = 0;
+  k->ws_wg = 0;
+  k->units = (N + 3) / 4; /* four wavefronts (draws in flight) per workgroup */
+  /* ms per draw on one workgroup of four wavefronts (size model) */
+  const double Md = (double)plan.wave_M;
+  k->cost = (double)N * 2.5e-5 * Md * (1.0 + Md / 64.0);
+  k->args.assign(reinterpret_cast<const unsigned char *>(&a),
+                 reinterpret_cast<const unsigned char *>(&a) + sizeof(a));
+  k->work_field = -1;
+  return SGDLM_OK;
+}
+
+int launch_lu_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                   double *logdet, hipStream_t st) {
+  const WaveArgs a = wave_args(S, N, theta, logdet);
   const long need = (N + 3) / 4; /* four wavefronts per workgroup */
   const int grid = (int)std::max(1L, std::min<long>(plan.wave_grid, need));
   const bool f64 = plan.prec == SGDLM_PREC_FP64;

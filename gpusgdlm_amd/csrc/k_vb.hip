@@ -58,8 +58,8 @@ __device__ double wave_max(double v) {
 }
 
 /* one workgroup: log-sum-exp weights */
-__global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
-                                                 double *w, double *stats) {
+__device__ __forceinline__ void weights_body(long N, const double *ld, double *w,
+                                             double *stats) {
   __shared__ double red[WTB / 64];
   __shared__ double bc;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -134,6 +134,17 @@ __global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
   }
 }
 
+__global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
+                                                 double *w, double *stats) {
+  weights_body(N, ld, w, stats);
+}
+
+/* k_weights of several models: one workgroup per model */
+__global__ __launch_bounds__(WTB) void k_weights_grp(const VbModel *__restrict__ tab) {
+  const VbModel &md = tab[blockIdx.x];
+  weights_body(md.N, md.logdet, md.w, md.stats);
+}
+
 /* One pass over the draws of chunk c for series j, on FP64 MFMA
  * (v_mfma_f64_16x16x4f64, p x N . N x p).  With x_k = sqrt(w lambda)
  * (theta_k - c_k) for k < p_j and x_{p_j} = sqrt(w lambda), X = sum x x^T
@@ -144,15 +155,14 @@ __global__ __launch_bounds__(WTB) void k_weights(long N, const double *ld,
  * Also sum w log lambda (part1).  NBLK = number of 16-wide coefficient
  * blocks, p_j + 1 <= 16 NBLK.  X goes to part2[(c m + j) (max_p+1)^2]. */
 template <int NBLK>
-__global__ __launch_bounds__(256) void k_mom(long N, int m, int max_p, int chunks,
-                                             const unsigned *p, const int *off, long P,
-                                             const double *w, const double *lam,
-                                             const double *theta, const double *m_t,
-                                             double *part1, double *part2) {
+__device__ __forceinline__ void mom_body(long N, int m, int max_p, int chunks,
+                                         const unsigned *p, const int *off, long P,
+                                         const double *w, const double *lam,
+                                         const double *theta, const double *m_t,
+                                         double *part1, double *part2, int j, int c) {
   constexpr int NT = NBLK * (NBLK + 1) / 2;
   __shared__ double tiles[NT][256];
   __shared__ double red[4];
-  const int j = blockIdx.y, c = blockIdx.x;
   const long i0 = N * c / chunks, i1 = N * (c + 1) / chunks;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -233,6 +243,30 @@ __global__ __launch_bounds__(256) void k_mom(long N, int m, int max_p, int chunk
     }
 }
 
+template <int NBLK>
+__global__ __launch_bounds__(256) void k_mom(long N, int m, int max_p, int chunks,
+                                             const unsigned *p, const int *off, long P,
+                                             const double *w, const double *lam,
+                                             const double *theta, const double *m_t,
+                                             double *part1, double *part2) {
+  mom_body<NBLK>(N, m, max_p, chunks, p, off, P, w, lam, theta, m_t, part1, part2,
+                 (int)blockIdx.y, (int)blockIdx.x);
+}
+
+/* k_mom of several models: blockIdx.y = series of all models, blockIdx.x =
+ * chunk (models with fewer chunks leave the high ones idle).  NBLK covers
+ * the largest max_p; the extra blocks only add exact zeros, so the moments
+ * equal the single-model launch's bit for bit. */
+template <int NBLK>
+__global__ __launch_bounds__(256) void k_mom_grp(const VbModel *__restrict__ tab,
+                                                 const int *__restrict__ ser_model) {
+  const VbModel &md = tab[ser_model[blockIdx.y]];
+  if ((int)blockIdx.x >= md.chunks)
+    return;
+  mom_body<NBLK>(md.N, md.m, md.max_p, md.chunks, md.p, md.off, md.P, md.w, md.lam, md.theta,
+                 md.m_t, md.part1, md.part2, (int)blockIdx.y - md.ser0, (int)blockIdx.x);
+}
+
 __device__ double psi_series(double x) {
   return log(x) - 1.0 / (2.0 * x) - 1.0 / (12.0 * x * x) +
          1.0 / (120.0 * x * x * x * x) - 1.0 / (252.0 * x * x * x * x * x * x) +
@@ -249,14 +283,14 @@ __device__ double psi1_series(double x) {
 }
 
 /* final per-series moment-match; one wavefront per series */
-__global__ __launch_bounds__(64) void k_vb_final(
+__device__ __forceinline__ void vb_final_body(
     int m, int max_p, int chunks, const unsigned *p, const double *m_t, const double *n_t,
     const double *s_t, const double *part1, const double *part2,
     double *mean_lambda, double *mean_log_lambda, double *mean_m,
     double *mean_C, double *mean_n, double *mean_s, double *mean_Q,
-    int *newton_iter) {
+    int *newton_iter, int j) {
   extern __shared__ double sh[];
-  const int j = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int pj = (int)p[j], mp = max_p, mp2 = max_p * max_p, mq = max_p + 1;
   double *V = sh;                     /* pj x pj (ld pj) */
   double *Aug = V + (size_t)pj * pj;  /* pj x 2pj        */
@@ -370,6 +404,40 @@ __global__ __launch_bounds__(64) void k_vb_final(
   }
 }
 
+__global__ __launch_bounds__(64) void k_vb_final(
+    int m, int max_p, int chunks, const unsigned *p, const double *m_t, const double *n_t,
+    const double *s_t, const double *part1, const double *part2,
+    double *mean_lambda, double *mean_log_lambda, double *mean_m,
+    double *mean_C, double *mean_n, double *mean_s, double *mean_Q,
+    int *newton_iter) {
+  vb_final_body(m, max_p, chunks, p, m_t, n_t, s_t, part1, part2, mean_lambda, mean_log_lambda,
+                mean_m, mean_C, mean_n, mean_s, mean_Q, newton_iter, (int)blockIdx.x);
+}
+
+/* k_vb_final over the series of several models */
+__global__ __launch_bounds__(64) void k_vb_final_grp(const VbModel *__restrict__ tab,
+                                                     const int *__restrict__ ser_model) {
+  const VbModel &md = tab[ser_model[blockIdx.x]];
+  vb_final_body(md.m, md.max_p, md.chunks, md.p, md.m_t, md.n_t, md.s_t, md.part1, md.part2,
+                md.mean_lambda, md.mean_log_lambda, md.mean_m, md.mean_C, md.mean_n, md.mean_s,
+                md.mean_Q, md.newton_iter, (int)blockIdx.x - md.ser0);
+}
+
+/* Before the LU of a step over several models: each model's flags[0]. */
+__global__ void k_grp_flags(const VbModel *__restrict__ tab, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    tab[i].lu_flags[0] = tab[i].flag_init;
+}
+
+/* After it: the weight statistics of every model side by side (one copy
+ * to the host). */
+__global__ void k_grp_stats(const VbModel *__restrict__ tab, int n, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 4 * n)
+    out[i] = tab[i / 4].stats[i % 4];
+}
+
 } // namespace
 
 int vb_chunks(long N) {
@@ -415,12 +483,52 @@ void launch_moments(const Structure &S, long N, const double *lam,
     break;
   }
   const size_t lds = (3 * (size_t)S.max_p * S.max_p + S.max_p) * sizeof(double);
-  (void)hipFuncSetAttribute((const void *)k_vb_final,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  (void)allow_dynamic_lds((const void *)k_vb_final, lds);
   hipLaunchKernelGGL(k_vb_final, dim3(S.m), dim3(64), lds, st, S.m, S.max_p,
                      chunks, S.p, m_t, n_t, s_t, B.part1, B.part2, B.mean_lambda,
                      B.mean_log_lambda, B.mean_m, B.mean_C, B.mean_n,
                      B.mean_s, B.mean_Q, B.newton_iter);
+}
+
+void launch_grp_flags(const VbModel *tab, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_grp_flags, dim3((n + 63) / 64), dim3(64), 0, st, tab, n);
+}
+
+void launch_weights_grp(const VbModel *tab, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_weights_grp, dim3(n), dim3(WTB), 0, st, tab);
+}
+
+int launch_moments_grp(const VbModel *tab, const int *ser_model, int n_series, int max_chunks,
+                       int max_p, hipStream_t st) {
+  if (n_series > 65535)
+    return SGDLM_ERR_ARG;
+  const int nblk = (max_p + 1 + 15) / 16;
+  const dim3 grid(max_chunks, n_series);
+  switch (nblk) {
+  case 1:
+    hipLaunchKernelGGL(k_mom_grp<1>, grid, dim3(256), 0, st, tab, ser_model);
+    break;
+  case 2:
+    hipLaunchKernelGGL(k_mom_grp<2>, grid, dim3(256), 0, st, tab, ser_model);
+    break;
+  case 3:
+    hipLaunchKernelGGL(k_mom_grp<3>, grid, dim3(256), 0, st, tab, ser_model);
+    break;
+  case 4:
+    hipLaunchKernelGGL(k_mom_grp<4>, grid, dim3(256), 0, st, tab, ser_model);
+    break;
+  default:
+    hipLaunchKernelGGL(k_mom_grp<5>, grid, dim3(256), 0, st, tab, ser_model);
+    break;
+  }
+  const size_t lds = (3 * (size_t)max_p * max_p + max_p) * sizeof(double);
+  (void)allow_dynamic_lds((const void *)k_vb_final_grp, lds);
+  hipLaunchKernelGGL(k_vb_final_grp, dim3(n_series), dim3(64), lds, st, tab, ser_model);
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
+void launch_grp_stats(const VbModel *tab, int n, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_grp_stats, dim3((4 * n + 255) / 256), dim3(256), 0, st, tab, n, out);
 }
 
 } // namespace sgdlm

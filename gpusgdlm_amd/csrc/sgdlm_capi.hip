@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -85,6 +86,9 @@ struct PendingEvent {
 struct LuTuned {
   int kind = SGDLM_LU_AUTO, threads = 0, nb = 0, max_front = -1;
   double tune_ms = 0.0, best_ms = 0.0;
+  /* time of one draw on one workgroup of the chosen kernel (the timed
+   * launch's time x its workgroups / draws; 0 = not measured) */
+  double draw_wg_ms = 0.0;
   int candidates = 0;
   bool from_cache = false;
 };
@@ -100,6 +104,95 @@ struct VbGraph {
 /* Tuned configurations of the process, by structure key. */
 std::mutex g_tune_mu;
 std::map<uint64_t, LuTuned> g_tune;
+
+std::atomic<uint64_t> g_ctx_ids{0};
+
+/* Streams a batch's LU launches of one stage run on concurrently (the
+ * process has four hardware queues by default). */
+constexpr size_t kGroupSideStreams = 4;
+/* Parts a batch is split into, each on a stream of its own. */
+constexpr int kBatchParts = 4;
+
+/* One LU launch over several models: the group form of a kernel, its
+ * launch shape, a model's argument struct for the by-value parameter and
+ * the device table / workgroup map it reads. */
+struct GrpLaunch {
+  const void *fn = nullptr;
+  int threads = 0, grid = 0;
+  size_t lds = 0;
+  std::vector<unsigned char> a0;
+  const void *tab = nullptr;
+  GrpMap gm;
+};
+
+/* A VB step of several models run as one launch per stage (one per kernel
+ * variant for the LU): the device tables it reads, built for one list of
+ * contexts and reused while nothing they bake in changes (sig). */
+struct VbGroup {
+  int n = 0, n_series = 0, max_p = 0, max_chunks = 0;
+  long max_N = 0;
+  void *blob = nullptr; /* model table, series map, epochs, stats, LU tables */
+  void *work = nullptr; /* LU workspace of every model */
+  VbModel *d_tab = nullptr;
+  int *d_ser = nullptr;
+  uint32_t *d_epochs = nullptr;
+  double *d_stats = nullptr;
+  uint32_t *h_epochs = nullptr; /* pinned */
+  double *h_stats = nullptr;    /* pinned, 4 per model */
+  std::vector<GrpLaunch> lu1, lu2; /* longest first */
+  /* the LU launches of a stage run concurrently on these streams, forked
+   * from and joined back to the leader's stream through the events */
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;
+};
+
+void drop_vb_group(VbGroup *g) {
+  if (!g)
+    return;
+  if (g->blob)
+    (void)hipFree(g->blob);
+  if (g->work)
+    (void)hipFree(g->work);
+  if (g->h_epochs)
+    (void)hipHostFree(g->h_epochs);
+  if (g->h_stats)
+    (void)hipHostFree(g->h_stats);
+  for (hipStream_t st : g->side)
+    (void)hipStreamDestroy(st);
+  for (hipEvent_t e : g->ev)
+    (void)hipEventDestroy(e);
+  delete g;
+}
+
+/* A batch: the models split into parts of about equal work, each a VbGroup
+ * whose stages run on a stream of its own, so that one part's draws and
+ * moments overlap another part's LU; captured and replayed as one graph in
+ * graph mode. */
+struct VbBatch {
+  std::vector<uintptr_t> sig;
+  std::vector<VbGroup *> parts;
+  std::vector<std::vector<int>> members; /* model indices of each part */
+  std::vector<hipStream_t> streams;      /* one per part (more than one part) */
+  std::vector<hipEvent_t> ev;            /* fork, then one join per part */
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+void drop_vb_batch(VbBatch *b) {
+  if (!b)
+    return;
+  if (b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if (b->graph)
+    (void)hipGraphDestroy(b->graph);
+  for (VbGroup *g : b->parts)
+    drop_vb_group(g);
+  for (hipStream_t st : b->streams)
+    (void)hipStreamDestroy(st);
+  for (hipEvent_t e : b->ev)
+    (void)hipEventDestroy(e);
+  delete b;
+}
 
 } // namespace
 
@@ -156,6 +249,14 @@ struct sgdlm_ctx {
   bool timing = false;
   std::map<std::string, std::pair<double, long>> times;
   std::vector<PendingEvent> pending;
+  /* process-unique identity (a batch's cached device tables are keyed by
+   * it, not by the address, which a later context may reuse) */
+  uint64_t id = 0;
+  /* the device tables of the last batch this context led (first in the
+   * list of sgdlm_batch_compute_vb_posterior) */
+  VbBatch *grp = nullptr;
+  long batch_group_calls = 0, batch_stream_calls = 0;
+  bool batch_no_group = false; /* sgdlm_set_batch_mode: one stream per context */
 };
 
 namespace {
@@ -704,10 +805,17 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
       ++n;
       if (ms < best_ms) {
         best_ms = ms;
+        const long wgs = p.wave_M > 0 ? std::min<long>(p.wave_grid, (nt + 3) / 4)
+                         : p.rs_MT > 0 ? p.rs_grid
+                         : p.nb        ? p.fast_grid
+                                       : p.pivot_grid;
+        best->draw_wg_ms = (double)ms * (double)std::min<long>(wgs, nt) / (double)nt;
         best->kind = k.kind;
         best->threads = k.threads;
         best->nb = k.nb;
-        best->max_front = mf;
+        /* the front limit the plan was built with: build_elimination halves
+         * a requested limit whose tables do not fit LDS */
+        best->max_front = c->max_front_used;
       }
     }
     if (rc)
@@ -930,6 +1038,576 @@ int vb_status(sgdlm_ctx *c) {
   return SGDLM_OK;
 }
 
+/* ---- a VB step of several models, one launch per stage ---- */
+
+/* The device buffers and LU plan run_vb would use for c (no allocation of
+ * the per-context LU workspace: the group brings its own). */
+int group_member(sgdlm_ctx *c, VbDev *D, LuPlan *plan) {
+  int rc;
+  if ((rc = ensure_sim_buffers(c)))
+    return rc;
+  if ((rc = vb_lu_plan(c, plan)))
+    return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
+  D->lam = SIM<double>(c, "lam");
+  D->theta = SIM<double>(c, "theta");
+  D->logdet = SIM<double>(c, "logdet");
+  D->lu_flags = SIM<int>(c, "lu_flags");
+  D->lu_work = nullptr;
+  D->B = vb_buffers(c);
+  return SGDLM_OK;
+}
+
+/* Builds the device tables of a step over the n contexts (all on one
+ * device, checked, with their buffers in place).  SGDLM_ERR_ARG: this list
+ * does not fit the group kernels (the caller runs the contexts one by one). */
+int build_vb_group(sgdlm_ctx *const *cs, int n, const std::vector<VbDev> &Ds,
+                   const std::vector<LuPlan> &plans, bool side_streams, VbGroup **out) {
+  sgdlm_ctx *c0 = cs[0];
+  VbGroup *g = new VbGroup();
+  g->n = n;
+  std::vector<VbModel> tab(n);
+  std::vector<int> ser;
+  for (int i = 0; i < n; ++i) {
+    sgdlm_ctx *c = cs[i];
+    const VbDev &D = Ds[i];
+    VbModel &v = tab[i];
+    v.m = c->S.m;
+    v.max_p = c->S.max_p;
+    v.chunks = D.B.chunks;
+    v.ser0 = (int)ser.size();
+    v.N = (long)c->nsim;
+    v.P = c->S.P;
+    v.seed = c->seed;
+    v.p = c->S.p;
+    v.off = c->S.off;
+    v.m_t = M_<double>(c, "m");
+    v.C = M_<double>(c, "C");
+    v.n_t = M_<double>(c, "n");
+    v.s_t = M_<double>(c, "s");
+    v.U = M_<double>(c, "U");
+    v.lam = D.lam;
+    v.theta = D.theta;
+    v.logdet = D.logdet;
+    v.lu_flags = D.lu_flags;
+    v.w = D.B.w;
+    v.stats = D.B.stats;
+    v.part1 = D.B.part1;
+    v.part2 = D.B.part2;
+    v.mean_lambda = D.B.mean_lambda;
+    v.mean_log_lambda = D.B.mean_log_lambda;
+    v.mean_m = D.B.mean_m;
+    v.mean_C = D.B.mean_C;
+    v.mean_n = D.B.mean_n;
+    v.mean_s = D.B.mean_s;
+    v.mean_Q = D.B.mean_Q;
+    v.newton_iter = D.B.newton_iter;
+    for (int j = 0; j < v.m; ++j)
+      ser.push_back(i);
+    g->max_p = std::max(g->max_p, v.max_p);
+    g->max_chunks = std::max(g->max_chunks, v.chunks);
+    g->max_N = std::max(g->max_N, v.N);
+  }
+  g->n_series = (int)ser.size();
+  auto give_up = [&](int rc) {
+    drop_vb_group(g);
+    return rc;
+  };
+  if (g->max_p > 32 || g->n_series > 65535)
+    return give_up(SGDLM_ERR_ARG);
+
+  /* LU stages: every model's kernels, grouped by kernel */
+  struct Item {
+    int model;
+    LuGrpKernel k;
+  };
+  std::vector<std::vector<Item>> st1, st2;
+  std::map<const void *, size_t> at1, at2;
+  for (int i = 0; i < n; ++i) {
+    LuGrpKernel k1, k2;
+    int init = 0;
+    /* the measured time of a draw, when the plan is the tuned one */
+    sgdlm_ctx *c = cs[i];
+    const bool tuned = !lu_pinned(c) && c->autotune && c->tuned_key == structure_key(c) &&
+                       c->tuned.max_front == c->max_front_used;
+    if (lu_grp_stages(c->S, plans[i], tab[i].N, tab[i].theta, tab[i].logdet, tab[i].lu_flags,
+                      tuned ? c->tuned.draw_wg_ms : 0.0, &init, &k1, &k2))
+      return give_up(SGDLM_ERR_ARG);
+    tab[i].flag_init = init;
+    auto add = [&](std::vector<std::vector<Item>> &st, std::map<const void *, size_t> &at,
+                   LuGrpKernel &k) {
+      auto it = at.find(k.fn);
+      if (it == at.end()) {
+        it = at.emplace(k.fn, st.size()).first;
+        st.emplace_back();
+      }
+      st[it->second].push_back({i, std::move(k)});
+    };
+    add(st1, at1, k1);
+    if (k2.fn)
+      add(st2, at2, k2);
+  }
+  /* workgroups per model within each launch: in proportion to its work
+   * (draws x time of a draw on one workgroup), so that every model's
+   * workgroups finish together, at least one and at most one per unit of
+   * work; the models with the slowest draws first in the grid.  The
+   * workspace of a model serves both stages. */
+  std::vector<size_t> ws(n, 0);
+  struct Shape {
+    int grid = 0, threads = 0;
+    size_t lds = 0;
+    double cost = 0.0;
+    std::vector<int> wg;
+  };
+  auto shape = [&](std::vector<Item> &items) {
+    std::stable_sort(items.begin(), items.end(), [&](const Item &a, const Item &b) {
+      return a.k.cost / (double)std::max(1L, tab[a.model].N) >
+             b.k.cost / (double)std::max(1L, tab[b.model].N);
+    });
+    Shape sh;
+    int per_cu = 1 << 20;
+    for (const Item &it : items) {
+      per_cu = std::min(per_cu, it.k.per_cu);
+      sh.cost += it.k.cost;
+      sh.lds = std::max(sh.lds, it.k.lds);
+      sh.threads = it.k.threads;
+    }
+    const double G = (double)c0->n_cu * per_cu;
+    const double per_wg = sh.cost > 0.0 ? sh.cost / G : 1.0;
+    for (const Item &it : items) {
+      long w = sh.cost > 0.0 ? (long)std::ceil(it.k.cost / per_wg - 1e-9) : 1;
+      w = std::max(1L, std::min(w, std::max(1L, it.k.units)));
+      sh.wg.push_back((int)w);
+      sh.grid += (int)w;
+      ws[it.model] = std::max(ws[it.model], (size_t)w * it.k.ws_wg);
+    }
+    return sh;
+  };
+  std::vector<Shape> sh1, sh2;
+  for (auto &items : st1)
+    sh1.push_back(shape(items));
+  for (auto &items : st2)
+    sh2.push_back(shape(items));
+  std::vector<size_t> ws_off(n, 0);
+  size_t work_bytes = 0;
+  for (int i = 0; i < n; ++i) {
+    ws_off[i] = work_bytes;
+    work_bytes += (ws[i] + 255) & ~(size_t)255;
+  }
+  if (work_bytes) {
+    if (hipMalloc(&g->work, work_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      g->work = nullptr;
+      drop_vb_group(g);
+      return fail(c0, SGDLM_ERR_ALLOC, "device allocation of %zu bytes for the batch LU "
+                                       "workspace failed", work_bytes);
+    }
+  }
+  /* the blob: model table, series map, epochs, stats, then per launch its
+   * argument table and workgroup map */
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_tab = take(sizeof(VbModel) * n), o_ser = take(sizeof(int) * ser.size()),
+               o_ep = take(sizeof(uint32_t) * n), o_st = take(sizeof(double) * 4 * n);
+  struct Pending {
+    std::vector<unsigned char> args;
+    std::vector<int> inst, base, count;
+    size_t o_args, o_inst, o_base, o_count;
+  };
+  std::vector<Pending> pend;
+  auto lay = [&](std::vector<Item> &items, const Shape &sh) {
+    Pending p;
+    const size_t asz = items[0].k.args.size();
+    int b = 0;
+    for (size_t q = 0; q < items.size(); ++q) {
+      Item &it = items[q];
+      if (it.k.args.size() != asz)
+        return false;
+      if (it.k.work_field >= 0) {
+        void *wp = static_cast<char *>(g->work) + ws_off[it.model];
+        std::memcpy(it.k.args.data() + it.k.work_field, &wp, sizeof(void *));
+      }
+      p.args.insert(p.args.end(), it.k.args.begin(), it.k.args.end());
+      p.base.push_back(b);
+      p.count.push_back(sh.wg[q]);
+      for (int w = 0; w < sh.wg[q]; ++w)
+        p.inst.push_back((int)q);
+      b += sh.wg[q];
+    }
+    p.o_args = take(p.args.size());
+    p.o_inst = take(sizeof(int) * p.inst.size());
+    p.o_base = take(sizeof(int) * p.base.size());
+    p.o_count = take(sizeof(int) * p.count.size());
+    pend.push_back(std::move(p));
+    return true;
+  };
+  for (size_t q = 0; q < st1.size(); ++q)
+    if (!lay(st1[q], sh1[q]))
+      return give_up(SGDLM_ERR_ARG);
+  for (size_t q = 0; q < st2.size(); ++q)
+    if (!lay(st2[q], sh2[q]))
+      return give_up(SGDLM_ERR_ARG);
+  std::vector<unsigned char> host(off, 0);
+  std::memcpy(host.data() + o_tab, tab.data(), sizeof(VbModel) * n);
+  std::memcpy(host.data() + o_ser, ser.data(), sizeof(int) * ser.size());
+  for (const Pending &p : pend) {
+    std::memcpy(host.data() + p.o_args, p.args.data(), p.args.size());
+    std::memcpy(host.data() + p.o_inst, p.inst.data(), sizeof(int) * p.inst.size());
+    std::memcpy(host.data() + p.o_base, p.base.data(), sizeof(int) * p.base.size());
+    std::memcpy(host.data() + p.o_count, p.count.data(), sizeof(int) * p.count.size());
+  }
+  if (hipMalloc(&g->blob, off) != hipSuccess) {
+    (void)hipGetLastError();
+    g->blob = nullptr;
+    drop_vb_group(g);
+    return fail(c0, SGDLM_ERR_ALLOC, "device allocation of %zu bytes for the batch tables failed",
+                off);
+  }
+  if (hipHostMalloc((void **)&g->h_epochs, sizeof(uint32_t) * n, hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&g->h_stats, sizeof(double) * 4 * n, hipHostMallocDefault) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    drop_vb_group(g);
+    return fail(c0, SGDLM_ERR_ALLOC, "pinned host allocation for the batch failed");
+  }
+  if (hipMemcpy(g->blob, host.data(), off, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    drop_vb_group(g);
+    return fail(c0, SGDLM_ERR_DEVICE, "upload of the batch tables failed");
+  }
+  char *base = static_cast<char *>(g->blob);
+  g->d_tab = reinterpret_cast<VbModel *>(base + o_tab);
+  g->d_ser = reinterpret_cast<int *>(base + o_ser);
+  g->d_epochs = reinterpret_cast<uint32_t *>(base + o_ep);
+  g->d_stats = reinterpret_cast<double *>(base + o_st);
+  size_t pi = 0;
+  auto mk = [&](std::vector<Item> &items, const Shape &sh, std::vector<GrpLaunch> &dst) {
+    const Pending &p = pend[pi++];
+    GrpLaunch L;
+    L.fn = items[0].k.fn;
+    L.threads = sh.threads;
+    L.grid = sh.grid;
+    L.lds = sh.lds;
+    L.a0 = items[0].k.args;
+    L.tab = base + p.o_args;
+    L.gm.inst = reinterpret_cast<const int *>(base + p.o_inst);
+    L.gm.base = reinterpret_cast<const int *>(base + p.o_base);
+    L.gm.count = reinterpret_cast<const int *>(base + p.o_count);
+    dst.push_back(std::move(L));
+  };
+  std::vector<double> c1, c2;
+  for (size_t q = 0; q < st1.size(); ++q) {
+    mk(st1[q], sh1[q], g->lu1);
+    c1.push_back(sh1[q].cost);
+  }
+  for (size_t q = 0; q < st2.size(); ++q) {
+    mk(st2[q], sh2[q], g->lu2);
+    c2.push_back(sh2[q].cost);
+  }
+  /* longest launch first */
+  auto order = [](std::vector<GrpLaunch> &L, const std::vector<double> &cost) {
+    std::vector<size_t> ix(L.size());
+    for (size_t q = 0; q < ix.size(); ++q)
+      ix[q] = q;
+    std::stable_sort(ix.begin(), ix.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<GrpLaunch> out;
+    for (size_t q : ix)
+      out.push_back(std::move(L[q]));
+    L = std::move(out);
+  };
+  order(g->lu1, c1);
+  order(g->lu2, c2);
+  /* side streams (default priority) and fork / join events */
+  const size_t n_side =
+      side_streams ? std::min<size_t>(kGroupSideStreams, std::max(g->lu1.size(), g->lu2.size()))
+                   : 0;
+  for (size_t q = 0; q < n_side; ++q) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      drop_vb_group(g);
+      return fail(c0, SGDLM_ERR_DEVICE, "batch stream creation failed");
+    }
+    g->side.push_back(st);
+  }
+  for (size_t q = 0; q < 2 * n_side + 2; ++q) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      drop_vb_group(g);
+      return fail(c0, SGDLM_ERR_DEVICE, "batch event creation failed");
+    }
+    g->ev.push_back(e);
+  }
+  *out = g;
+  return SGDLM_OK;
+}
+
+int launch_grp_lu(sgdlm_ctx *c, const GrpLaunch &L, hipStream_t st) {
+  if (allow_dynamic_lds(L.fn, L.lds) != hipSuccess)
+    return fail(c, SGDLM_ERR_DEVICE, "batch LU: LDS attribute refused (%zu bytes)", L.lds);
+  GrpMap gm = L.gm;
+  const void *tab = L.tab;
+  void *args[] = {const_cast<unsigned char *>(L.a0.data()), &tab, &gm};
+  if (hipLaunchKernel(L.fn, dim3(L.grid), dim3(L.threads), args, L.lds, st) != hipSuccess)
+    return fail(c, SGDLM_ERR_DEVICE, "batch LU launch failed: %s",
+                hipGetErrorString(hipGetLastError()));
+  return SGDLM_OK;
+}
+
+/* The launches of one LU stage, concurrently: forked from c's stream onto
+ * the group's side streams (round robin, longest first) and joined back. */
+int launch_grp_stage(sgdlm_ctx *c, VbGroup *g, const std::vector<GrpLaunch> &Ls,
+                     hipStream_t st) {
+  int rc;
+  if (Ls.size() <= 1 || g->side.empty()) {
+    for (const GrpLaunch &L : Ls)
+      if ((rc = launch_grp_lu(c, L, st)))
+        return rc;
+    return SGDLM_OK;
+  }
+  const size_t ns = std::min(g->side.size(), Ls.size());
+  HIPCHK(c, hipEventRecord(g->ev[0], st));
+  for (size_t q = 0; q < ns; ++q)
+    HIPCHK(c, hipStreamWaitEvent(g->side[q], g->ev[0], 0));
+  for (size_t q = 0; q < Ls.size(); ++q)
+    if ((rc = launch_grp_lu(c, Ls[q], g->side[q % ns])))
+      return rc;
+  for (size_t q = 0; q < ns; ++q) {
+    HIPCHK(c, hipEventRecord(g->ev[1 + q], g->side[q]));
+    HIPCHK(c, hipStreamWaitEvent(st, g->ev[1 + q], 0));
+  }
+  return SGDLM_OK;
+}
+
+/* The stages of a group step on stream st (capturable: kernels and one
+ * device-to-host copy of the statistics); stage times on c's timers when
+ * t_on (st is c's stream). */
+int enqueue_vb_group(sgdlm_ctx *c, VbGroup *g, hipStream_t st, bool t_on) {
+  int rc;
+  {
+    StageTimer t(c, "batch_draw", t_on);
+    launch_cholesky_grp(g->d_tab, g->d_ser, g->n_series, g->max_p, st);
+    if ((rc = launch_draw_grp(g->d_tab, g->d_ser, g->d_epochs, g->n_series, g->max_N, g->max_p,
+                              st)))
+      return fail(c, rc, "batch draw launch failed");
+  }
+  CHECK_LAUNCH(c);
+  {
+    StageTimer t(c, "batch_lu", t_on);
+    launch_grp_flags(g->d_tab, g->n, st);
+    if ((rc = launch_grp_stage(c, g, g->lu1, st)) || (rc = launch_grp_stage(c, g, g->lu2, st)))
+      return rc;
+  }
+  {
+    StageTimer t(c, "batch_weights", t_on);
+    launch_weights_grp(g->d_tab, g->n, st);
+  }
+  CHECK_LAUNCH(c);
+  {
+    StageTimer t(c, "batch_moments", t_on);
+    if ((rc = launch_moments_grp(g->d_tab, g->d_ser, g->n_series, g->max_chunks, g->max_p, st)))
+      return fail(c, rc, "batch moment launch failed");
+  }
+  launch_grp_stats(g->d_tab, g->n, g->d_stats, st);
+  CHECK_LAUNCH(c);
+  HIPCHK(c, hipMemcpyAsync(g->h_stats, g->d_stats, sizeof(double) * 4 * g->n,
+                           hipMemcpyDeviceToHost, st));
+  return SGDLM_OK;
+}
+
+/* Every part of a batch: one part on c0's stream, or each part on its own
+ * stream, forked from c0's stream and joined back. */
+int enqueue_vb_batch(sgdlm_ctx *c0, VbBatch *b, bool t_on) {
+  int rc;
+  if (b->parts.size() == 1)
+    return enqueue_vb_group(c0, b->parts[0], c0->stream, t_on);
+  HIPCHK(c0, hipEventRecord(b->ev[0], c0->stream));
+  for (size_t q = 0; q < b->parts.size(); ++q) {
+    HIPCHK(c0, hipStreamWaitEvent(b->streams[q], b->ev[0], 0));
+    if ((rc = enqueue_vb_group(c0, b->parts[q], b->streams[q], false)))
+      return rc;
+    HIPCHK(c0, hipEventRecord(b->ev[1 + q], b->streams[q]));
+    HIPCHK(c0, hipStreamWaitEvent(c0->stream, b->ev[1 + q], 0));
+  }
+  return SGDLM_OK;
+}
+
+/* The parts of a batch of n models: up to kBatchParts, at least four
+ * models each, by a greedy split of the estimated work (the LU's measured
+ * or modelled time plus the draws and moments, which scale with m x N). */
+std::vector<std::vector<int>> split_batch(sgdlm_ctx *const *cs, int n,
+                                          const std::vector<LuPlan> &plans) {
+  const int P = std::max(1, std::min(kBatchParts, n / 4));
+  std::vector<double> w(n);
+  for (int i = 0; i < n; ++i) {
+    sgdlm_ctx *c = cs[i];
+    const LuPlan &p = plans[i];
+    const double N = (double)c->nsim;
+    const bool tuned = !lu_pinned(c) && c->autotune && c->tuned_key == structure_key(c) &&
+                       c->tuned.max_front == c->max_front_used && c->tuned.draw_wg_ms > 0.0;
+    const double wgs = p.wave_M > 0 ? p.wave_grid : p.rs_MT > 0 ? p.rs_grid
+                       : p.nb        ? p.fast_grid
+                                     : p.pivot_grid;
+    const double Md = (double)std::max(p.fast_M, 16);
+    const double lu = tuned ? N * c->tuned.draw_wg_ms / std::max(1.0, wgs)
+                            : N * 1e-9 * Md * Md * Md / 6.0;
+    w[i] = lu + 2e-7 * N * (double)c->S.m;
+  }
+  std::vector<int> ix(n);
+  for (int i = 0; i < n; ++i)
+    ix[i] = i;
+  std::stable_sort(ix.begin(), ix.end(), [&](int a, int b) { return w[a] > w[b]; });
+  std::vector<std::vector<int>> parts(P);
+  std::vector<double> load(P, 0.0);
+  for (int i : ix) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    parts[q].push_back(i);
+    load[q] += w[i];
+  }
+  for (auto &pt : parts)
+    std::sort(pt.begin(), pt.end());
+  return parts;
+}
+
+int build_vb_batch(sgdlm_ctx *const *cs, int n, const std::vector<VbDev> &Ds,
+                   const std::vector<LuPlan> &plans, std::vector<uintptr_t> sig, VbBatch **out) {
+  sgdlm_ctx *c0 = cs[0];
+  VbBatch *b = new VbBatch();
+  b->sig = std::move(sig);
+  b->members = split_batch(cs, n, plans);
+  const bool multi = b->members.size() > 1;
+  for (const auto &mem : b->members) {
+    std::vector<sgdlm_ctx *> pc;
+    std::vector<VbDev> pd;
+    std::vector<LuPlan> pp;
+    for (int i : mem) {
+      pc.push_back(cs[i]);
+      pd.push_back(Ds[i]);
+      pp.push_back(plans[i]);
+    }
+    VbGroup *g = nullptr;
+    if (build_vb_group(pc.data(), (int)pc.size(), pd, pp, !multi, &g)) {
+      drop_vb_batch(b);
+      return SGDLM_ERR_ARG;
+    }
+    b->parts.push_back(g);
+  }
+  if (multi) {
+    for (size_t q = 0; q < b->parts.size(); ++q) {
+      hipStream_t st = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        drop_vb_batch(b);
+        return fail(c0, SGDLM_ERR_DEVICE, "batch stream creation failed");
+      }
+      b->streams.push_back(st);
+    }
+    for (size_t q = 0; q < b->parts.size() + 1; ++q) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        drop_vb_batch(b);
+        return fail(c0, SGDLM_ERR_DEVICE, "batch event creation failed");
+      }
+      b->ev.push_back(e);
+    }
+  }
+  *out = b;
+  return SGDLM_OK;
+}
+
+/* One VB step of every context in cs as one launch per stage (per part)
+ * on the first context's stream.  SGDLM_ERR_ARG before anything ran: the
+ * list does not fit the group path (different devices, a pending LU
+ * autotuning, draw bucket or grid limits); the caller then runs the
+ * contexts one by one.  Per-context statuses go to rcs. */
+int run_vb_group(sgdlm_ctx *const *cs, int n, std::vector<int> &rcs) {
+  sgdlm_ctx *c0 = cs[0];
+  for (int i = 0; i < n; ++i)
+    if (cs[i]->device != c0->device || tuning_pending(cs[i]) || cs[i]->S.max_p > 32)
+      return SGDLM_ERR_ARG;
+  std::vector<VbDev> Ds(n);
+  std::vector<LuPlan> plans(n);
+  std::vector<uintptr_t> sig;
+  for (int i = 0; i < n; ++i) {
+    sgdlm_ctx *c = cs[i];
+    if (group_member(c, &Ds[i], &plans[i]) || check_epochs(c, 1))
+      return SGDLM_ERR_ARG; /* the one-by-one path reports it per context */
+    std::vector<uintptr_t> s1 = vb_signature(c, (long)c->nsim, Ds[i], plans[i]);
+    sig.push_back((uintptr_t)c->id);
+    sig.insert(sig.end(), s1.begin(), s1.end());
+    sig.push_back((uintptr_t)plans[i].rs_lds);
+  }
+  VbBatch *b = c0->grp;
+  if (!b || b->sig != sig) {
+    drop_vb_batch(c0->grp);
+    c0->grp = nullptr;
+    if (build_vb_batch(cs, n, Ds, plans, sig, &b))
+      return SGDLM_ERR_ARG;
+    c0->grp = b;
+  }
+  for (size_t q = 0; q < b->parts.size(); ++q) {
+    VbGroup *g = b->parts[q];
+    for (size_t k = 0; k < b->members[q].size(); ++k)
+      g->h_epochs[k] = ++cs[b->members[q][k]]->epoch;
+    HIPCHK(c0, hipMemcpyAsync(g->d_epochs, g->h_epochs, sizeof(uint32_t) * g->n,
+                              hipMemcpyHostToDevice, c0->stream));
+  }
+  /* the step runs as one captured graph (replayed while the tables stay),
+   * so that its parts' streams run concurrently; eagerly only for stage
+   * timing */
+  int rc;
+  if (!c0->timing) {
+    if (!b->exec) {
+      HIPCHK(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeRelaxed));
+      const int erc = enqueue_vb_batch(c0, b, false);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(c0->stream, &gr);
+      if (erc || e != hipSuccess) {
+        if (gr)
+          (void)hipGraphDestroy(gr);
+        (void)hipGetLastError();
+        return erc ? erc : fail(c0, SGDLM_ERR_DEVICE, "batch graph capture failed: %s",
+                                hipGetErrorString(e));
+      }
+      hipGraphExec_t x = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        (void)hipGetLastError();
+        return fail(c0, SGDLM_ERR_DEVICE, "batch graph instantiation failed: %s",
+                    hipGetErrorString(ei));
+      }
+      b->graph = gr;
+      b->exec = x;
+      ++c0->graph_captures;
+    }
+    HIPCHK(c0, hipGraphLaunch(b->exec, c0->stream));
+    ++c0->graph_replays;
+  } else if ((rc = enqueue_vb_batch(c0, b, c0->timing))) {
+    return rc;
+  }
+  if ((rc = sync_and_collect(c0)))
+    return rc;
+  for (size_t q = 0; q < b->parts.size(); ++q) {
+    for (size_t k = 0; k < b->members[q].size(); ++k) {
+      sgdlm_ctx *c = cs[b->members[q][k]];
+      std::memcpy(c->h_stats, b->parts[q]->h_stats + 4 * k, 4 * sizeof(double));
+      c->lu_flags_valid = true;
+      c->vb_valid = true;
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    rcs[i] = vb_status(cs[i]);
+  ++c0->batch_group_calls;
+  return SGDLM_OK;
+}
+
 } // namespace
 
 /* ======================================================================== */
@@ -982,6 +1660,7 @@ int sgdlm_create(sgdlm_ctx **out, size_t no_gpus) {
     delete c;
     return SGDLM_ERR_ALLOC;
   }
+  c->id = ++g_ctx_ids;
   *out = c;
   return SGDLM_OK;
 }
@@ -991,6 +1670,8 @@ int sgdlm_destroy(sgdlm_ctx *c) {
     return SGDLM_OK;
   (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
+  drop_vb_batch(c->grp);
+  c->grp = nullptr;
   free_pool(c->sim);
   free_pool(c->mem);
   for (auto &pe : c->pending) {
@@ -1487,6 +2168,38 @@ int sgdlm_batch_compute_vb_posterior(sgdlm_ctx *const *ctxs, size_t count, int *
     return SGDLM_ERR_ARG;
   int first = SGDLM_OK;
   std::vector<int> rcs(count, SGDLM_OK);
+  /* one launch per stage over every model, when they all fit it */
+  bool ready = count > 1 && count < (1u << 20);
+  for (size_t i = 0; i < count && ready; ++i) {
+    sgdlm_ctx *c = ctxs[i];
+    ready = c && !check_init(c) && !check_parents(c) && !check_sim(c) && !check_prior(c, false) &&
+            !c->batch_no_group;
+  }
+  if (ready) {
+    const int rc = run_vb_group(ctxs, (int)count, rcs);
+    if (rc == SGDLM_OK) {
+      for (size_t i = 0; i < count; ++i) {
+        if (status)
+          status[i] = rcs[i];
+        if (rcs[i] && !first)
+          first = rcs[i];
+      }
+      return first;
+    }
+    if (rc != SGDLM_ERR_ARG) {
+      /* the group step failed after it started: every model reports it */
+      for (size_t i = 0; i < count; ++i) {
+        if (ctxs[i] != ctxs[0])
+          ctxs[i]->err = ctxs[0]->err;
+        if (status)
+          status[i] = rc;
+      }
+      return rc;
+    }
+    std::fill(rcs.begin(), rcs.end(), SGDLM_OK);
+  }
+  if (count && ctxs[0])
+    ++ctxs[0]->batch_stream_calls;
   /* enqueue every context's stages on its own stream, then wait once each */
   for (size_t i = 0; i < count; ++i) {
     sgdlm_ctx *c = ctxs[i];
@@ -1778,6 +2491,23 @@ int sgdlm_get_graph_stats(const sgdlm_ctx *c, long *captures, long *replays) {
     *captures = c->graph_captures;
   if (replays)
     *replays = c->graph_replays;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_batch_mode(sgdlm_ctx *c, int mode) {
+  if (!c || (mode != SGDLM_BATCH_GROUP && mode != SGDLM_BATCH_STREAMS))
+    return SGDLM_ERR_ARG;
+  c->batch_no_group = mode == SGDLM_BATCH_STREAMS;
+  return SGDLM_OK;
+}
+
+int sgdlm_get_batch_stats(const sgdlm_ctx *c, long *group_calls, long *stream_calls) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (group_calls)
+    *group_calls = c->batch_group_calls;
+  if (stream_calls)
+    *stream_calls = c->batch_stream_calls;
   return SGDLM_OK;
 }
 

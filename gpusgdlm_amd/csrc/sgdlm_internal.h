@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/sgdlm.h"
 
 namespace sgdlm {
@@ -49,6 +51,58 @@ struct Structure {
             *e_rb_sg_idx = nullptr;
   const uint32_t *e_rb_pairs = nullptr;
 };
+
+/* Workgroup map of one launch over several independent models: workgroup b
+ * works for model inst[b] as that model's workgroup b - base[inst[b]] of
+ * count[inst[b]] (device arrays). */
+struct GrpMap {
+  const int *inst = nullptr, *base = nullptr, *count = nullptr;
+};
+
+/* A kernel's arguments, local workgroup index and workgroup count: its own
+ * (GRP = false) or, in a launch over several models, those of the model
+ * this workgroup serves (tab[inst]). */
+template <bool GRP, typename A>
+__device__ __forceinline__ A grp_args(const A &a0, const A *__restrict__ tab, const GrpMap &gm,
+                                      int &bid, int &nbk) {
+  if constexpr (GRP) {
+    const int i = gm.inst[blockIdx.x];
+    bid = (int)blockIdx.x - gm.base[i];
+    nbk = gm.count[i];
+    return tab[i];
+  } else {
+    (void)tab;
+    (void)gm;
+    bid = (int)blockIdx.x;
+    nbk = (int)gridDim.x;
+    return a0;
+  }
+}
+
+/* One model of a VB step run over several independent models at once
+ * (device table entry; the kernels of every stage read it by model index). */
+struct VbModel {
+  int m = 0, max_p = 0, chunks = 0;
+  int ser0 = 0;                  /* index of its series 0 among all models' */
+  long N = 0, P = 0;
+  uint64_t seed = 0;
+  const unsigned *p = nullptr;
+  const int *off = nullptr;
+  const double *m_t = nullptr, *C = nullptr, *n_t = nullptr, *s_t = nullptr;
+  double *U = nullptr, *lam = nullptr, *theta = nullptr, *logdet = nullptr;
+  int *lu_flags = nullptr;       /* flags[0] = flag_init before the LU      */
+  int flag_init = 0;
+  /* VbBuffers of the model */
+  double *w = nullptr, *stats = nullptr, *part1 = nullptr, *part2 = nullptr;
+  double *mean_lambda = nullptr, *mean_log_lambda = nullptr, *mean_m = nullptr,
+         *mean_C = nullptr, *mean_n = nullptr, *mean_s = nullptr, *mean_Q = nullptr;
+  int *newton_iter = nullptr;
+};
+
+/* Lets kernel fn take up to lds bytes of dynamic LDS (the device's largest
+ * per workgroup, set once per kernel and process, so launches need no
+ * attribute call of their own). */
+hipError_t allow_dynamic_lds(const void *fn, size_t lds);
 
 /* ---- filter (k_filter.hip) ---- */
 void launch_posterior(const Structure &S, double *m_t, double *C_t,
@@ -158,6 +212,34 @@ int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *t
 int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
                      const double *theta, void *work, double *logdet,
                      int *flags, hipStream_t st);
+/* One model's kernel of an LU stage in a launch over several models (its
+ * group form reads the model's argument struct from a table and serves it
+ * with the workgroups a GrpMap assigns). */
+struct LuGrpKernel {
+  const void *fn = nullptr;  /* group form of the kernel                   */
+  int threads = 0;           /* workgroup size                             */
+  int per_cu = 1;            /* workgroups resident per CU                 */
+  size_t lds = 0;            /* dynamic LDS bytes of this model's layout   */
+  size_t ws_wg = 0;          /* workspace bytes per workgroup (0: none)    */
+  long units = 0;            /* most workgroups worth assigning            */
+  double cost = 0.0;         /* work of the model: workgroup-ms            */
+  std::vector<unsigned char> args; /* the kernel's argument struct         */
+  long work_field = -1;      /* byte offset of its workspace pointer in args */
+};
+/* The LU stages of one model's log-determinants in a launch over several
+ * models: *flag_init is stored to flags[0] first (the count the fallback
+ * kernel reads), then `first` runs (one-wavefront, register, fast or
+ * pivot-everything kernel) and, when second->fn is set, the
+ * partial-pivoting kernel on the draws `first` flagged.  cost = N x the
+ * time of one draw on one workgroup in ms: draw_wg_ms when measured (> 0),
+ * else a size model. */
+int lu_grp_stages(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                  double *logdet, int *flags, double draw_wg_ms, int *flag_init,
+                  LuGrpKernel *first, LuGrpKernel *second);
+int lu_grp_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
+              double *logdet, int *flags, LuGrpKernel *k);
+int lu_grp_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                double *logdet, LuGrpKernel *k);
 /* flags: N + 1 ints of scratch, as for launch_lu_logdet. */
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
@@ -184,6 +266,18 @@ void launch_weights(long N, const double *logdet, double *w, double *stats,
 void launch_moments(const Structure &S, long N, const double *lam,
                     const double *theta, const double *m_t, const double *n_t,
                     const double *s_t, const VbBuffers &B, hipStream_t st);
+
+/* ---- a VB step over several models (one launch per stage; tab: device
+ * table of n models, ser_model: model of each of the n_series series) ---- */
+void launch_cholesky_grp(const VbModel *tab, const int *ser_model, int n_series, int max_p,
+                         hipStream_t st);
+int launch_draw_grp(const VbModel *tab, const int *ser_model, const uint32_t *epochs,
+                    int n_series, long max_N, int max_p, hipStream_t st);
+void launch_grp_flags(const VbModel *tab, int n, hipStream_t st);
+void launch_weights_grp(const VbModel *tab, int n, hipStream_t st);
+int launch_moments_grp(const VbModel *tab, const int *ser_model, int n_series, int max_chunks,
+                       int max_p, hipStream_t st);
+void launch_grp_stats(const VbModel *tab, int n, double *out, hipStream_t st);
 
 } // namespace sgdlm
 

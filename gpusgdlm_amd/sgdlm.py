@@ -403,6 +403,20 @@ class SGDLM:
         self._call("sgdlm_get_graph_stats", C.byref(cap), C.byref(rep))
         return {"captures": cap.value, "replays": rep.value}
 
+    def set_batch_mode(self, mode: str = "group"):
+        """How a batch_vb led by this model (first in the list) runs: "group"
+        (one launch per stage over all models) or "streams" (each model's
+        stages on its own stream)."""
+        modes = {"group": nat.BATCH_GROUP, "streams": nat.BATCH_STREAMS}
+        self._call("sgdlm_set_batch_mode", C.c_int(modes[mode]))
+
+    def batch_stats(self):
+        """Batches this model led: as one launch per stage, as one stream per
+        model."""
+        g, s = C.c_long(), C.c_long()
+        self._call("sgdlm_get_batch_stats", C.byref(g), C.byref(s))
+        return {"group": g.value, "streams": s.value}
+
     def set_lu_autotune(self, on: bool = True):
         """Time the feasible VB LU configurations on the first VB call of a
         structure and keep the fastest (default on)."""
@@ -450,8 +464,10 @@ class SGDLM:
 
 def batch_vb(models, nsim, nsim_batch=None):
     """computeVBPosterior (results left on the device) for a list of SGDLM
-    objects in one batched call: the GPU overlaps the instances.  Returns the
-    list of per-model status codes; results via get_vb_result()."""
+    objects in one batched call: every stage is one launch over all models
+    (or, with set_batch_mode("streams") on the first model, each model's
+    stages on its own stream).  Returns the list of per-model status codes;
+    results via get_vb_result()."""
     lib = nat.load()
     nb = int(nsim_batch or nsim)
     for g in models:

@@ -182,15 +182,33 @@ int sgdlm_compute_vb_posterior_noalloc(sgdlm_ctx *ctx, size_t nsim,
                                        size_t *scratch_bytes);
 /* Batched VB for many independent models (one context each, all on this
  * GPU): runs sgdlm_compute_vb_posterior with device-resident results on
- * ctxs[0..count), enqueueing every context's stages on its own stream before
- * waiting for any, so the device overlaps small instances; fetch results
- * with sgdlm_get_vb_result.  status (optional, count entries) receives each
+ * ctxs[0..count); fetch results with sgdlm_get_vb_result.  Every stage runs
+ * as one launch over all models and their draws (the LU as one launch per
+ * kernel variant the models' plans use, workgroups split by work) on the
+ * first context's stream, from device tables built on the first call for
+ * this list and kept by ctxs[0] while the contexts' structures, buffers,
+ * seeds and LU plans stay the same.  The models are split into up to four
+ * parts of about equal work whose stages run on streams of their own, and
+ * the step is captured once as a hipGraph and replayed (run eagerly only
+ * while ctxs[0] has stage timing on).  Results equal the
+ * one-call-per-model results bit for bit.  A list the group path cannot
+ * take (contexts on different devices, max_p > 32, more than 65535 series
+ * in all, an LU structure not tuned yet, sgdlm_set_batch_mode(ctxs[0],
+ * SGDLM_BATCH_STREAMS)) runs one context per stream instead, enqueued
+ * before any wait.  status (optional, count entries) receives each
  * context's status; the return value is the first non-OK status.  Contexts
  * must not be used concurrently from other threads during the call.  Serves
- * the reference's many-instance use (one HostWrapper per model) without a
- * host round trip per model. */
+ * the reference's many-instance use (one HostWrapper per model,
+ * HostWrapperImpl.cu:1059-1228 once per model) without a host round trip
+ * or a launch per model. */
 int sgdlm_batch_compute_vb_posterior(sgdlm_ctx *const *ctxs, size_t count,
                                      int *status);
+#define SGDLM_BATCH_GROUP 0   /* one launch per stage over all models      */
+#define SGDLM_BATCH_STREAMS 1 /* each model's stages on its own stream     */
+/* How a batch led by ctx (ctxs[0]) runs.  No counterpart in the reference. */
+int sgdlm_set_batch_mode(sgdlm_ctx *ctx, int mode);
+/* Batches ctx led as one launch per stage, and one stream per model. */
+int sgdlm_get_batch_stats(const sgdlm_ctx *ctx, long *group_calls, long *stream_calls);
 /* Per-stage device timing with HIP events recorded on the context's stream
  * around each launch (enable = 1 resets the counters).  stage: "lu" (VB
  * log-det LU), "draw", "weights", "moments", "forecast_lu", "filter". */

@@ -172,9 +172,9 @@ def test_forecast_m1000_quantiles_vs_oracle(hiplib, c1000):
 
 # ------------------------------------------------ many independent models
 def test_batched_instances_vs_oracle(hiplib):
-    """8 models with m log-uniform on [20, 400] through one
-    sgdlm_batch_compute_vb_posterior call (FP32 LU), each against the
-    oracle's VB of the same model."""
+    """8 models with m log-uniform on [20, 400] through
+    sgdlm_batch_compute_vb_posterior (FP32 LU, one launch per stage over all
+    models), each against the oracle's VB of the same model."""
     T, N = 100, 10000
     mods = instance_models(8, T=T + 2, seed=77)
     print("instances m =", [md.m for md in mods], flush=True)
@@ -187,7 +187,13 @@ def test_batched_instances_vs_oracle(hiplib):
         filter_steps(g, mod, 1, T)
         g.computePosterior(mod.y[T], mod.F(T))
         gs.append(g)
+    # the first batched call tunes each structure's LU (one stream per
+    # model); the step compared runs as one launch per stage over all models
     assert batch_vb(gs, N) == [0] * len(gs)
+    for i, g in enumerate(gs):
+        g.set_rng_state(SEED + i, 0)
+    assert batch_vb(gs, N) == [0] * len(gs)
+    assert gs[0].batch_stats()["group"] == 1
     for i, (g, mod) in enumerate(zip(gs, mods)):
         o = orc.OracleSGDLM(seed=SEED + i, epoch=0)
         configure(o, mod)

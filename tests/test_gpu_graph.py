@@ -92,6 +92,12 @@ def test_graph_batched_instances_match_direct(hiplib):
         assert batch_vb(graph, N) == [0] * len(mods)
         for i, (a, b) in enumerate(zip(direct, graph)):
             _same(a.get_vb_result(), b.get_vb_result(), f"instance {i} step {step}")
-    assert all(g.graph_stats()["replays"] >= 3 for g in graph)
+    # the first call tunes each structure's LU (one stream per model); the
+    # others run as one launch per stage, replayed from one captured graph
+    # (batches always run as a graph)
+    st = graph[0].batch_stats()
+    print(f"  leader batch stats {st}, graph {graph[0].graph_stats()}", flush=True)
+    assert st["group"] == 3 and direct[0].batch_stats()["group"] == 3
+    assert graph[0].graph_stats()["replays"] >= 3 and direct[0].graph_stats()["replays"] >= 3
     for g in direct + graph:
         g.close()
