@@ -1437,12 +1437,15 @@ int enqueue_vb_batch(sgdlm_ctx *c0, VbBatch *b, bool t_on) {
   return SGDLM_OK;
 }
 
-/* The parts of a batch of n models: up to kBatchParts, at least four
- * models each, by a greedy split of the estimated work (the LU's measured
+/* The parts of a batch of n models: up to kBatchParts (or
+ * SGDLM_BATCH_PARTS from the environment), at least four models each, by a greedy split of the estimated work (the LU's measured
  * or modelled time plus the draws and moments, which scale with m x N). */
 std::vector<std::vector<int>> split_batch(sgdlm_ctx *const *cs, int n,
                                           const std::vector<LuPlan> &plans) {
-  const int P = std::max(1, std::min(kBatchParts, n / 4));
+  int want = kBatchParts;
+  if (const char *e = std::getenv("SGDLM_BATCH_PARTS"))
+    want = std::max(1, std::min(16, std::atoi(e)));
+  const int P = std::max(1, std::min(want, n / 4));
   std::vector<double> w(n);
   for (int i = 0; i < n; ++i) {
     sgdlm_ctx *c = cs[i];
