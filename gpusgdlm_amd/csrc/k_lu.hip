@@ -1708,6 +1708,24 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     plan->nb = 0;
     plan->rs_MT = 0;
   }
+  /* dense blocks after the prefix up to 128 rows: one wavefront per draw on
+   * the Schur complement */
+  plan->wsch_M = 0;
+  plan->wsch_grid = 0;
+  plan->wsch_lds = 0;
+  if (want_kind == SGDLM_LU_WAVE_PREFIX) {
+    const size_t lds = solve ? 0 : lu_wsch_lds(S, prec);
+    if (!lds || lds > lds_max)
+      return SGDLM_ERR_ARG;
+    const int per_cu = lu_wsch_per_cu(S, prec, lds);
+    if (per_cu < 1)
+      return SGDLM_ERR_ARG;
+    plan->wsch_M = S.e_wb_M;
+    plan->wsch_lds = lds;
+    plan->wsch_grid = std::max(1, n_cu) * per_cu;
+    plan->nb = 0;
+    plan->rs_MT = 0;
+  }
   plan->work_bytes =
       (size_t)std::max(plan->pivot_grid, std::max(plan->grid, plan->fast_grid)) * M * M * ts;
   if (plan->slot_stride) {
@@ -1744,6 +1762,8 @@ hipError_t allow_dynamic_lds(const void *fn, size_t lds) {
 }
 
 int lu_plan_kind(const LuPlan &plan) {
+  if (plan.wsch_M > 0)
+    return SGDLM_LU_WAVE_PREFIX;
   if (plan.wave_M > 0)
     return SGDLM_LU_WAVE;
   if (plan.rs_MT > 0)
@@ -1786,6 +1806,17 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
     if (set_int(flags, 0, st))
       return SGDLM_ERR_DEVICE;
     return launch_lu_wave(S, plan, N, theta, logdet, st);
+  }
+  if (plan.wsch_M > 0) {
+    /* one wavefront per draw after the prefix; draws whose prefix fails go
+     * to the partial-pivoting kernel */
+    if (set_int(flags, 0, st))
+      return SGDLM_ERR_DEVICE;
+    const int rc = launch_lu_wsch(S, plan, N, theta, logdet, flags, st);
+    if (rc)
+      return rc;
+    set_pivot_layout(a, S, plan);
+    return dispatch_pivot<false>(a, plan, st);
   }
   if (plan.nb == 0 && plan.rs_MT == 0) {
     /* no fast configuration: partial pivoting for every draw (all of them
@@ -1888,6 +1919,11 @@ int lu_grp_stages_shape(const Structure &S, const LuPlan &plan, long N, const do
   if (plan.wave_M > 0) {
     *flag_init = 0;
     return lu_grp_wave(S, plan, N, theta, logdet, first);
+  }
+  if (plan.wsch_M > 0) {
+    *flag_init = 0;
+    const int rc = lu_grp_wsch(S, plan, N, theta, logdet, flags, first);
+    return rc ? rc : pivot_stage(false, second);
   }
   if (plan.nb == 0 && plan.rs_MT == 0) {
     *flag_init = (int)N;

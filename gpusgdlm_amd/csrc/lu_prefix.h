@@ -40,14 +40,28 @@ struct PrefixTabs {
   const int *slot_init, *tab; /* global copies */
 };
 
+/* The barrier between the elimination's phases: the workgroup's, or (WAVE:
+ * one wavefront eliminating a draw of its own, tid = its lane, nthreads =
+ * 64) the wavefront's LDS ordering. */
+template <bool WAVE> __device__ __forceinline__ void prefix_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 /* Loads the prefix tables into LDS tb[] (when load_tab), fills the slots
  * sl[] of one draw (theta of the draw: th) and eliminates the prefix wave by
  * wave: per wave the multipliers l_ik = a_ik / a_kk replace a_ik, then every
  * target slot subtracts its gathered sum of l_ik * a_kj (fixed order, no
  * atomics).  Adds log|a_kk| of the prefix pivots to ls (FP64); a zero pivot
  * or a multiplier beyond lbound (NaN included) sets bad and *eflag; amax
- * tracks max |l|.  Ends with a workgroup barrier. */
-template <typename T>
+ * tracks max |l|.  Ends with a workgroup barrier (WAVE: the wavefront's LDS
+ * ordering, with load_tab false and the tables already in LDS). */
+template <typename T, bool WAVE = false>
 __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixTabs &p, int tid,
                                  int nthreads, bool load_tab, int &bad, T &amax, double &ls,
                                  int *eflag, T lbound) {
@@ -91,7 +105,7 @@ __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixT
         sl[s] = x[u];
     }
   }
-  __syncthreads();
+  prefix_sync<WAVE>();
   const int *mptr = tb + p.o_mptr, *tptr = tb + p.o_tptr, *cptr = tb + p.o_cptr;
   const unsigned *mult = reinterpret_cast<const unsigned *>(tb + p.o_mult);
   const unsigned *contrib = reinterpret_cast<const unsigned *>(tb + p.o_contrib);
@@ -115,7 +129,7 @@ __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixT
       if (bad)
         *eflag = 1;
     }
-    __syncthreads();
+    prefix_sync<WAVE>();
     const int t1 = tptr[w + 1];
     for (int t = tptr[w] + tid; t < t1; t += nthreads) {
       const int s = tslot[t];
@@ -127,7 +141,7 @@ __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixT
       }
       sl[s] = acc;
     }
-    __syncthreads();
+    prefix_sync<WAVE>();
   }
   const int *piv = tb + p.o_piv;
   for (int q = tid; q < p.n1; q += nthreads) {
@@ -138,7 +152,7 @@ __device__ void prefix_eliminate(T *sl, int *tb, const double *th, const PrefixT
     }
     ls += log(fabs((double)pv));
   }
-  __syncthreads();
+  prefix_sync<WAVE>();
 }
 
 } // namespace sgdlm

@@ -469,6 +469,40 @@ template <typename V> int upload(sgdlm_ctx *c, const char *name, const V &v) {
   return SGDLM_OK;
 }
 
+/* Banded Schur tables of the one-wavefront LU after the prefix
+ * (k_lu_wave.hip k_lu_wsch): bands of 16 rows, row stride M + 4 for the
+ * M x M block (M = the dense block rounded up to 32, 64 or 128).  Leaves
+ * e_wb_M = 0 without a prefix or beyond 128 rows. */
+int build_wave_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
+  Structure &S = c->S;
+  S.e_wb_M = 0;
+  const int M = lu_wave_dim(P.M2, SGDLM_PREC_FP32);
+  if (P.n1 == 0 || P.M2 == 0 || M == 0)
+    return SGDLM_OK;
+  const int stride = M + 4;
+  BandPlan B;
+  build_band_plan(P, 16, stride, M, B);
+  int rc;
+  if ((rc = upload(c, "wb_blk_ptr", B.blk_ptr)) || (rc = upload(c, "wb_st_off", B.st_off)) ||
+      (rc = upload(c, "wb_pairs", B.st_pairs)) || (rc = upload(c, "wb_brounds", B.brounds)) ||
+      (rc = upload(c, "wb_bcbase", B.bcbase)) || (rc = upload(c, "wb_sg_ptr", B.sg_ptr)) ||
+      (rc = upload(c, "wb_sg_off", B.sg_off)) || (rc = upload(c, "wb_sg_idx", B.sg_idx)))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  S.e_wb_blk_ptr = M_<int>(c, "wb_blk_ptr");
+  S.e_wb_st_off = M_<int>(c, "wb_st_off");
+  S.e_wb_pairs = M_<uint32_t>(c, "wb_pairs");
+  S.e_wb_brounds = M_<int>(c, "wb_brounds");
+  S.e_wb_bcbase = M_<int>(c, "wb_bcbase");
+  S.e_wb_sg_ptr = M_<int>(c, "wb_sg_ptr");
+  S.e_wb_sg_off = M_<int>(c, "wb_sg_off");
+  S.e_wb_sg_idx = M_<int>(c, "wb_sg_idx");
+  S.e_wb_stride = stride;
+  S.e_wb_nband = B.n_band;
+  S.e_wb_M = M;
+  return SGDLM_OK;
+}
+
 /* LDS layout and banded Schur tables of the register-resident LU
  * (k_lu_rs.hip) for plan P: the prefix slots and tables, two flags and the
  * reduction words, then one region shared by the staged panel tiles and
@@ -597,6 +631,9 @@ int build_elimination(sgdlm_ctx *c, const std::vector<GammaEntry> &ents, int fro
   S.e_bwd_ptr = M_<int>(c, "e_bwd_ptr");
   S.e_bwd = M_<uint32_t>(c, "e_bwd");
   ++c->gen;
+  int rc2;
+  if ((rc2 = build_wave_band_tables(c, P)))
+    return rc2;
   return build_band_tables(c, P);
 }
 
@@ -780,6 +817,9 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
     std::vector<Cand> cs = cands;
     if (wave_ok && mf == c->max_front_default)
       cs.push_back({SGDLM_LU_WAVE, 0, 0});
+    /* dense blocks up to 128 rows after this front's prefix */
+    if (c->S.e_wb_M > 0)
+      cs.push_back({SGDLM_LU_WAVE_PREFIX, 0, 0});
     for (const Cand &k : cs) {
       LuPlan p;
       if (lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, k.threads, k.nb, &p, k.kind))
@@ -805,8 +845,9 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
       ++n;
       if (ms < best_ms) {
         best_ms = ms;
-        const long wgs = p.wave_M > 0 ? std::min<long>(p.wave_grid, (nt + 3) / 4)
-                         : p.rs_MT > 0 ? p.rs_grid
+        const long wgs = p.wave_M > 0   ? std::min<long>(p.wave_grid, (nt + 3) / 4)
+                         : p.wsch_M > 0 ? std::min<long>(p.wsch_grid, (nt + 3) / 4)
+                         : p.rs_MT > 0  ? p.rs_grid
                          : p.nb        ? p.fast_grid
                                        : p.pivot_grid;
         best->draw_wg_ms = (double)ms * (double)std::min<long>(wgs, nt) / (double)nt;
@@ -948,7 +989,8 @@ std::vector<uintptr_t> vb_signature(sgdlm_ctx *c, long N, const VbDev &D, const 
           (uintptr_t)p.threads, (uintptr_t)p.fast_grid, (uintptr_t)p.lds, (uintptr_t)p.grid,
           (uintptr_t)p.rs_MT, (uintptr_t)p.rs_grid, (uintptr_t)p.rs_lds, (uintptr_t)p.pivot_nb,
           (uintptr_t)p.pivot_grid, (uintptr_t)p.pivot_lds, (uintptr_t)p.pivot_schur_M,
-          (uintptr_t)p.work_bytes, (uintptr_t)p.wave_M, (uintptr_t)p.wave_grid};
+          (uintptr_t)p.work_bytes, (uintptr_t)p.wave_M, (uintptr_t)p.wave_grid,
+          (uintptr_t)p.wsch_M, (uintptr_t)p.wsch_grid, (uintptr_t)p.wsch_lds};
 }
 
 /* Graph mode: the VB step of run_vb captured once as a hipGraph (re-captured
@@ -1453,9 +1495,11 @@ std::vector<std::vector<int>> split_batch(sgdlm_ctx *const *cs, int n,
     const double N = (double)c->nsim;
     const bool tuned = !lu_pinned(c) && c->autotune && c->tuned_key == structure_key(c) &&
                        c->tuned.max_front == c->max_front_used && c->tuned.draw_wg_ms > 0.0;
-    const double wgs = p.wave_M > 0 ? p.wave_grid : p.rs_MT > 0 ? p.rs_grid
-                       : p.nb        ? p.fast_grid
-                                     : p.pivot_grid;
+    const double wgs = p.wave_M > 0   ? p.wave_grid
+                       : p.wsch_M > 0 ? p.wsch_grid
+                       : p.rs_MT > 0  ? p.rs_grid
+                       : p.nb         ? p.fast_grid
+                                      : p.pivot_grid;
     const double Md = (double)std::max(p.fast_M, 16);
     const double lu = tuned ? N * c->tuned.draw_wg_ms / std::max(1.0, wgs)
                             : N * 1e-9 * Md * Md * Md / 6.0;
@@ -2433,15 +2477,18 @@ int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, in
   int rc = vb_lu_plan(c, &plan);
   if (rc)
     return fail(c, rc, "no LU configuration for threads=%d nb=%d", c->lu_threads, c->lu_nb);
-  const bool reg = plan.rs_MT > 0, wave = plan.wave_M > 0;
+  const bool reg = plan.rs_MT > 0, wave = plan.wave_M > 0, wsch = plan.wsch_M > 0;
   if (grid)
-    *grid = wave ? plan.wave_grid : reg ? plan.rs_grid : (plan.nb ? plan.fast_grid : plan.grid);
+    *grid = wsch   ? plan.wsch_grid
+            : wave ? plan.wave_grid
+            : reg  ? plan.rs_grid
+                   : (plan.nb ? plan.fast_grid : plan.grid);
   if (lds)
-    *lds = wave ? 0 : reg ? plan.rs_lds : plan.lds;
+    *lds = wsch ? plan.wsch_lds : wave ? 0 : reg ? plan.rs_lds : plan.lds;
   if (nb)
-    *nb = wave ? 0 : reg ? 16 : plan.nb;
+    *nb = (wave || wsch) ? 0 : reg ? 16 : plan.nb;
   if (threads)
-    *threads = wave ? 256 : reg ? 512 : (plan.nb ? plan.threads : 0);
+    *threads = (wave || wsch) ? 256 : reg ? 512 : (plan.nb ? plan.threads : 0);
   return SGDLM_OK;
 }
 
@@ -2458,9 +2505,10 @@ int sgdlm_get_lu_kernel(const sgdlm_ctx *cc, int *kind, int *dense_dim, int *piv
   if (kind)
     *kind = lu_plan_kind(plan);
   if (dense_dim)
-    *dense_dim = plan.wave_M > 0 ? plan.wave_M
-                 : plan.rs_MT > 0 ? 16 * plan.rs_MT
-                                  : (plan.nb ? plan.fast_M : c->S.M);
+    *dense_dim = plan.wsch_M > 0   ? plan.wsch_M
+                 : plan.wave_M > 0 ? plan.wave_M
+                 : plan.rs_MT > 0  ? 16 * plan.rs_MT
+                                   : (plan.nb ? plan.fast_M : c->S.M);
   if (pivot_dim)
     *pivot_dim = plan.wave_M > 0 ? plan.wave_M
                  : plan.pivot_schur_M > 0 ? plan.pivot_schur_M : c->S.M;
@@ -2470,7 +2518,7 @@ int sgdlm_get_lu_kernel(const sgdlm_ctx *cc, int *kind, int *dense_dim, int *piv
 int sgdlm_set_lu_kernel(sgdlm_ctx *c, int kind) {
   if (!c)
     return SGDLM_ERR_ARG;
-  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_WAVE)
+  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_WAVE_PREFIX)
     return fail(c, SGDLM_ERR_ARG, "unknown LU kernel kind %d", kind);
   c->lu_kind = kind;
   return SGDLM_OK;

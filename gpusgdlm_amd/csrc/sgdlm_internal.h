@@ -50,6 +50,14 @@ struct Structure {
             *e_rb_bcbase = nullptr, *e_rb_sg_ptr = nullptr, *e_rb_sg_off = nullptr,
             *e_rb_sg_idx = nullptr;
   const uint32_t *e_rb_pairs = nullptr;
+  /* The same for the one-wavefront LU after the prefix (k_lu_wsch): bands
+   * of 16 rows, row stride e_wb_stride, dense block e_wb_M = M2 rounded up
+   * to 32, 64 or 128 (0: not available). */
+  int e_wb_M = 0, e_wb_stride = 0, e_wb_nband = 0;
+  const int *e_wb_blk_ptr = nullptr, *e_wb_st_off = nullptr, *e_wb_brounds = nullptr,
+            *e_wb_bcbase = nullptr, *e_wb_sg_ptr = nullptr, *e_wb_sg_off = nullptr,
+            *e_wb_sg_idx = nullptr;
+  const uint32_t *e_wb_pairs = nullptr;
 };
 
 /* Workgroup map of one launch over several independent models: workgroup b
@@ -163,6 +171,11 @@ struct LuPlan {
   /* small models: one wavefront per draw over the whole matrix in
    * registers, wave_M rows (k_lu_wave.hip); 0 = not used */
   int wave_M = 0, wave_grid = 0;
+  /* dense blocks after the prefix: one wavefront per draw on the wsch_M
+   * rows of the Schur complement (k_lu_wsch); 0 = not used.  Flagged draws
+   * go to the partial-pivoting kernel. */
+  int wsch_M = 0, wsch_grid = 0;
+  size_t wsch_lds = 0;
   size_t work_bytes = 0;      /* global workspace bytes                      */
   /* register-resident LU of the dense block (k_lu_rs.hip) instead of the
    * fast kernel: rs_MT > 0 */
@@ -187,6 +200,14 @@ int lu_wave_dim(int M, int prec);
 int lu_wave_grid(int wave_M, int prec, int n_cu);
 int launch_lu_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
                    double *logdet, hipStream_t st);
+/* The one-wavefront LU after the sparse prefix (k_lu_wave.hip k_lu_wsch):
+ * its dynamic LDS for a structure at precision prec (0: it does not apply)
+ * and resident workgroups per CU; the launch appends the draws whose prefix
+ * fails to flags. */
+size_t lu_wsch_lds(const Structure &S, int prec);
+int lu_wsch_per_cu(const Structure &S, int prec, size_t lds);
+int launch_lu_wsch(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                   double *logdet, int *flags, hipStream_t st);
 /* *p = v on stream st by a one-thread kernel (capturable in a graph,
  * unlike a memset node). */
 int set_device_int(int *p, int v, hipStream_t st);
@@ -240,6 +261,8 @@ int lu_grp_rs(const Structure &S, const LuPlan &plan, long N, const double *thet
               double *logdet, int *flags, LuGrpKernel *k);
 int lu_grp_wave(const Structure &S, const LuPlan &plan, long N, const double *theta,
                 double *logdet, LuGrpKernel *k);
+int lu_grp_wsch(const Structure &S, const LuPlan &plan, long N, const double *theta,
+                double *logdet, int *flags, LuGrpKernel *k);
 /* flags: N + 1 ints of scratch, as for launch_lu_logdet. */
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,

@@ -55,6 +55,11 @@ extern "C" {
 #define SGDLM_LU_WAVE 4     /* small models (m <= 128 FP32, <= 64 FP64): one
                                wavefront per draw, the whole matrix in
                                registers, partial pivoting                  */
+#define SGDLM_LU_WAVE_PREFIX 5 /* dense blocks after the sparse prefix of up
+                               to 128 rows (FP32) / 64 (FP64): one wavefront
+                               per draw, the prefix on its own LDS slots, the
+                               Schur complement in registers, partial
+                               pivoting                                     */
 
 typedef struct sgdlm_ctx sgdlm_ctx;
 

@@ -151,3 +151,38 @@ def test_wave_kernel_small_models_vs_oracle(hiplib, m, prec, T):
         kappa = np.linalg.cond(_dense(mod, th[:, :, i]))
         print(f"  draw {i}: kappa {kappa:.2e} err {err[i]:.2e}", flush=True)
         assert err[i] <= (1e-15 if prec == "fp64" else 6e-8) * kappa
+
+
+@pytest.mark.parametrize("m,k,prec,T", [(100, 8, "fp32", 40), (120, 12, "fp32", 3),
+                                        (160, 16, "fp32", 40), (140, 14, "bf16x3", 40),
+                                        (90, 10, "fp64", 40), (100, 14, "fp64", 3)])
+def test_wave_prefix_kernel_vs_oracle(hiplib, m, k, prec, T):
+    """One wavefront per draw after the sparse prefix: the prefix on the
+    wavefront's LDS slots (multipliers up to 64), the Schur complement of
+    48..112 rows gathered into registers, partial pivoting on it.  Bars as
+    for the whole-matrix wavefront kernel; draws whose prefix fails go to
+    the partial-pivoting kernel (counted as fallback draws)."""
+    mod = random_model(m, k, T=T + 2, seed=70 + m + k)
+    N = 512
+    print(f"wave-prefix kernel m={m} k={k} {prec} T={T} N={N}", flush=True)
+    r = _gpu(mod, prec, N, lambda g: g.set_lu_kernel("wave_prefix"), t_post=T)
+    print(f"  {r['cfg']} elimination {r['el']}", flush=True)
+    assert r["cfg"]["kernel"] == "wave_prefix" and r["cfg"]["dense_dim"] in (32, 64, 128)
+    assert 0 < r["el"]["n_dense"] <= r["cfg"]["dense_dim"]
+    ld = _oracle_logdets(mod, N, t_post=T)
+    o = orc.OracleSGDLM(seed=SEED, epoch=0)
+    configure(o, mod)
+    filter_steps(o, mod, 1, T)
+    o.computePosterior(mod.y[T], mod.F(T))
+    prm = o.getParameters()
+    lam, th = orc.sample(mod.m, mod.max_p, mod.p, prm["m"], prm["C"], prm["n"], prm["s"], N, SEED, 1)
+    err = np.abs(r["lw"] - ld)
+    easy = 1e-10 if prec == "fp64" else 5e-3
+    hard = np.nonzero(~(err <= easy))[0]
+    print(f"  |err| median {np.median(err):.2e} max {err.max():.2e}; {len(hard)} above {easy:g}",
+          flush=True)
+    assert len(hard) <= N // 20
+    for i in hard:
+        kappa = np.linalg.cond(_dense(mod, th[:, :, i]))
+        print(f"  draw {i}: kappa {kappa:.2e} err {err[i]:.2e}", flush=True)
+        assert err[i] <= (1e-15 if prec == "fp64" else 6e-8) * kappa
