@@ -86,6 +86,8 @@ SIGNATURES = {
     "sgdlm_set_lu_max_front": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_batch_compute_vb_posterior": (C.c_int, [C.POINTER(_ctx), C.c_size_t, C.POINTER(C.c_int)]),
     "sgdlm_set_batch_mode": (C.c_int, [_ctx, C.c_int]),
+    "sgdlm_get_lu_solve_kernel": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
+    "sgdlm_set_lu_solve_kernel": (C.c_int, [_ctx, C.c_int]),
     "sgdlm_get_batch_stats": (C.c_int, [_ctx, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "sgdlm_get_lu_elimination": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.POINTER(C.c_long)]),

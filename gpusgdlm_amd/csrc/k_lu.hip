@@ -1331,6 +1331,18 @@ __global__ void k_set_int(int *p, int v) {
     *p = v;
 }
 
+/* *a = 0 (op 0), *a += *b (op 1), *a = *b (op 2): the fallback count over
+ * the chunks of a solve, kept by the device. */
+__global__ void k_int_op(int *a, const int *b, int op) {
+  if (threadIdx.x == 0)
+    *a = op == 0 ? 0 : op == 1 ? *a + *b : *b;
+}
+
+int int_op(int *a, const int *b, int op, hipStream_t st) {
+  k_int_op<<<1, 64, 0, st>>>(a, b, op);
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
 int set_int(int *p, int v, hipStream_t st) {
   k_set_int<<<1, 64, 0, st>>>(p, v);
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
@@ -1688,14 +1700,30 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
    * register-resident kernel, unless another kind or a fast-kernel shape
    * was asked for */
   plan->rs_MT = 0;
-  if (!solve && prec != SGDLM_PREC_FP64 && !want_threads && !want_nb &&
+  plan->rs_chunk = 0;
+  if ((solve ? prec == SGDLM_PREC_FP32 : prec != SGDLM_PREC_FP64) && !want_threads && !want_nb &&
       (want_kind == SGDLM_LU_AUTO || want_kind == SGDLM_LU_REGISTER) && S.e_rb_rows > 0) {
     plan->rs_MT = S.e_rb_tiles;
     plan->rs_grid = plan->grid;
     plan->rs_lds = S.e_rb_lds;
     plan->rs_panel_base = S.e_rb_panel_base;
     plan->rs_flag_base = S.e_rb_flag_base;
+    if (solve) {
+      /* forecast solves: factors of up to rs_chunk draws in the workspace
+       * (at most 4 GB; a whole number of the factor kernel's rounds) */
+      plan->rs_sol_stride = rs_solve_stride(S, plan->rs_MT);
+      const size_t rounds = std::max<size_t>(
+          1, ((size_t)4 << 30) / (plan->rs_sol_stride * 4) / (size_t)plan->rs_grid);
+      plan->rs_chunk = (int)(std::min<size_t>(rounds, 64) * (size_t)plan->rs_grid);
+      if (lu_subst_shape(S, plan->rs_MT, n_cu, &plan->subst_grid, &plan->subst_lds) ||
+          plan->subst_lds > lds_max) {
+        plan->rs_MT = 0;
+        plan->rs_chunk = 0;
+      }
+    }
   }
+  if (solve && want_kind == SGDLM_LU_PIVOTING)
+    plan->nb = 0;
   if (want_kind == SGDLM_LU_REGISTER && !plan->rs_MT)
     return SGDLM_ERR_ARG;
   /* small models: one wavefront per draw on the whole matrix */
@@ -1731,6 +1759,11 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   if (plan->slot_stride) {
     plan->slot_save_off = (plan->work_bytes + 255) & ~(size_t)255;
     plan->work_bytes = plan->slot_save_off + (size_t)plan->fast_grid * plan->slot_stride;
+  }
+  if (plan->rs_chunk) {
+    plan->rs_sol_off = (plan->work_bytes + 255) & ~(size_t)255;
+    plan->work_bytes = plan->rs_sol_off + (size_t)plan->rs_chunk * plan->rs_sol_stride * 4 +
+                       (size_t)plan->rs_chunk * 4;
   }
   return SGDLM_OK;
 }
@@ -1846,6 +1879,40 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
                     double *y, int refine, int *flags, hipStream_t st) {
+  if (plan.rs_chunk > 0) {
+    /* register kernel: per chunk, factors written out, one wavefront per
+     * draw substitutes, flagged draws solved with partial pivoting; the
+     * fallback count summed over the chunks in flags[N + 1] */
+    float *sol = reinterpret_cast<float *>(static_cast<char *>(work) + plan.rs_sol_off);
+    int *ok = reinterpret_cast<int *>(sol + (size_t)plan.rs_chunk * plan.rs_sol_stride);
+    int *total = flags + N + 1;
+    if (int_op(total, nullptr, 0, st))
+      return SGDLM_ERR_DEVICE;
+    /* equal chunks, each a whole number of the factor kernel's rounds */
+    const long nch = (N + plan.rs_chunk - 1) / plan.rs_chunk;
+    const long g = std::max(1, plan.rs_grid);
+    const long per = std::min<long>(plan.rs_chunk, ((N + nch - 1) / nch + g - 1) / g * g);
+    for (long c0 = 0; c0 < N; c0 += per) {
+      const long n = std::min<long>(per, N - c0);
+      const double *th = theta + (size_t)c0 * S.P;
+      int rc;
+      if (set_int(flags, 0, st))
+        return SGDLM_ERR_DEVICE;
+      if ((rc = launch_lu_rs_factor(S, plan, n, th, sol, ok, flags, st)) ||
+          (rc = launch_lu_subst(S, plan, n, th, rhs + (size_t)c0 * S.m, y + (size_t)c0 * S.m, sol,
+                                ok, refine, st)))
+        return rc;
+      LuArgs a = base_args(S, n, th, work);
+      a.rhs = rhs + (size_t)c0 * S.m;
+      a.y = y + (size_t)c0 * S.m;
+      a.refine = refine;
+      a.flag_count = flags;
+      a.flag_list = flags + 1;
+      if ((rc = dispatch_pivot<true>(a, plan, st)) || int_op(total, flags, 1, st))
+        return rc ? rc : SGDLM_ERR_DEVICE;
+    }
+    return int_op(flags, total, 2, st);
+  }
   LuArgs a = base_args(S, N, theta, work);
   a.rhs = rhs;
   a.y = y;

@@ -70,6 +70,15 @@ struct RsArgs {
   const int *blk_ptr, *st_off, *brounds, *bcbase, *sg_ptr, *sg_off, *sg_idx;
   const unsigned *st_pairs;
   unsigned zero_pair;
+  /* solves (SOLVE): per draw, at sol + d sol_stride, the MT x MT tiles of
+   * the factors (tile (I, J) at (I MT + J) 256 floats, row-major: L_IJ
+   * below, U_IJ above the diagonal, the diagonal tile packed with L_kk^-1
+   * strictly below and U_kk^-1 on and above its diagonal), then the prefix
+   * slots; sol_ok[d] = 1 for a factored draw, 0 for one left to the
+   * partial-pivoting kernel */
+  float *sol;
+  long sol_stride;
+  int *sol_ok;
 };
 
 __device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
@@ -110,6 +119,19 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Workgroup barrier that orders LDS only: global stores still in flight
+ * (the factors a solve writes out, never read back by the workgroup) are
+ * not waited for.  The log-determinant kernels keep __syncthreads. */
+template <bool LDS_ONLY> __device__ __forceinline__ void step_sync() {
+  if constexpr (LDS_ONLY) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  } else {
+    __syncthreads();
+  }
 }
 
 /* LDS pointers (address space 3): the out-of-line diagonal factorization
@@ -215,7 +237,7 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
   }
 }
 
-template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool GRP>
+template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool GRP, bool SOLVE = false>
 __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
     RsArgs a0, const RsArgs *__restrict__ tab, GrpMap gm) {
   constexpr int NW = WR * WC, NT = 64 * NW;
@@ -270,6 +292,16 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
     prefix_eliminate<float>(sl, tb, th, a.pre, tid, NT, !a.tab_persist || d == (long)bid, bad,
                             amax, ls, flagA, kMaxMultiplier);
     bool stop = *flagA != 0;
+    float *sol_d = nullptr;
+    if constexpr (SOLVE) {
+      /* the prefix factors for the substitutions */
+      sol_d = a.sol + (size_t)d * a.sol_stride;
+      if (!stop) {
+        float *so = sol_d + (size_t)MT * MT * 256;
+        for (int q = tid; q < a.pre.ns; q += NT)
+          so[q] = sl[q];
+      }
+    }
 
     f32x4 acc[RT][CT];
 #pragma unroll
@@ -420,7 +452,7 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
     if (!stop)
       phase1(0);
     for (int k = 0; k < MT && !stop; ++k) {
-      __syncthreads();
+      step_sync<SOLVE>();
       if (*flagA) {
         stop = true;
         break;
@@ -429,6 +461,18 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
       if (wr == k % WR) {
         const f32x4 li = *reinterpret_cast<const f32x4 *>(Linvb + frag);
         const int ik = k / WR;
+        if constexpr (SOLVE) {
+          if (wc == k % WC) {
+            /* the diagonal tile's inverse factors, packed: L_kk^-1 below,
+             * U_kk^-1 on and above the diagonal (element (4g + r, l16)) */
+            float *t = sol_d + ((size_t)k * MT + k) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = 4 * g + r;
+              t[row * 16 + l16] = row > l16 ? Linvb[row * TR + l16] : UinvTb[l16 * TR + row];
+            }
+          }
+        }
 #pragma unroll
         for (int ia = 0; ia < RT; ++ia) {
           if (ia == ik) {
@@ -494,7 +538,7 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
       }
       if (bad)
         *flagB = 1;
-      __syncthreads();
+      step_sync<SOLVE>();
       if (*flagB) {
         stop = true;
         break;
@@ -506,11 +550,41 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
         phase1(k + 1);
         update(k, [&](int J) { return J > k + 1; });
       }
+      if constexpr (SOLVE) {
+        /* step k's U_kJ (LDS, transposed) and L_Ik (LDS, negated) tiles to
+         * the draw's factors, row-major, before the next step rewrites them;
+         * by the wavefronts other than the one that factored tile k + 1 (the
+         * longest phase 4) */
+        const int nt = MT - k - 1;
+        const int dw = (k + 1) % WR + WR * ((k + 1) % WC);
+        const int pw = wave < dw ? wave : wave - 1;
+        if (nt > 0 && wave != dw) {
+          /* four consecutive columns of one row per lane: one 16-byte store */
+          for (int e = pw * 64 + lane; e < 2 * nt * 64; e += NT - 64) {
+            const int t = e >> 6, row = (e >> 2) & 15, col = 4 * (e & 3);
+            f32x4 v;
+            float *dst;
+            if (t < nt) {
+              const int J = k + 1 + t;
+              const float *src = Ub + J * TSF + col * TR + row;
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                v[i] = src[i * TR];
+              dst = sol_d + (k * MT + J) * 256 + row * 16 + col;
+            } else {
+              const int I = k + 1 + t - nt;
+              v = -*reinterpret_cast<const f32x4 *>(Lb + I * TSF + row * TR + col);
+              dst = sol_d + (I * MT + k) * 256 + row * 16 + col;
+            }
+            *reinterpret_cast<f32x4 *>(dst) = v;
+          }
+        }
+      }
     }
 
     /* ---- log|det| (FP64, fixed reduction order) and the fallback list */
-    __syncthreads();
-    if (!stop)
+    step_sync<SOLVE>();
+    if (!stop && !SOLVE)
       for (int q = tid; q < 16 * MT; q += NT)
         ls += log((double)pivb[q]);
 #pragma unroll
@@ -527,9 +601,225 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
         const int slot = atomicAdd(a.flag_count, 1);
         a.flag_list[slot] = (int)d;
       }
-      a.logdet[d] = stop ? NAN : logdet;
+      if (a.logdet)
+        a.logdet[d] = stop ? NAN : logdet;
+      if constexpr (SOLVE)
+        a.sol_ok[d] = stop ? 0 : 1;
     }
     __syncthreads();
+  }
+}
+
+/* Forecast substitutions from the factors the register kernel wrote
+ * (SOLVE): y = (I - Gamma)^{-1} rhs for every draw with sol_ok set, one
+ * wavefront per draw, FP64 accumulation.  Prefix forward (wave by wave,
+ * the saved slots), the dense part as block rows of 16 (forward: t_K =
+ * b_K - sum_{J<K} L_KJ y_J, y_K = L_KK^-1 t_K; backward with U and
+ * U_KK^-1), prefix backward; with refine one more pass on the FP64
+ * residual rhs - (I - Gamma) y (sparse Gamma in series order), its
+ * correction added.  Lane (r, q) = (lane % 16, lane / 16) takes row r of a
+ * block and columns 4q .. 4q + 3 of each tile (one 16-byte load); the four
+ * column groups are summed by shuffles in a fixed order. */
+struct SubstArgs {
+  long n_draws;
+  int m, MT;
+  long P;
+  const double *theta, *rhs;
+  double *y;
+  const float *sol;
+  long sol_stride;
+  const int *sol_ok;
+  int refine;
+  int n1, n2, waves;
+  const int *wave_q, *node_of, *fwd_ptr, *bwd_ptr, *piv;
+  const unsigned *fwd, *bwd;
+  const int *g_rowptr, *g_col, *g_idx;
+  int wave_bytes; /* LDS per wavefront: x (n1 + 16 MT doubles), 16 doubles */
+};
+
+__device__ __forceinline__ void wave_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* sum over tiles J0 <= J < J1 of row (r) of tile J, columns 4q .. 4q + 3,
+ * times x_J's entries 4q .. 4q + 3 (row: the lane's row of tile J0 = 0
+ * of the block row; x: x_0 + 4q); eight (then four) tiles' loads issued
+ * together so a wavefront keeps several 1 KB reads in flight; fixed
+ * summation order */
+__device__ __forceinline__ double tile_row_dot(const float *row, const double *x, int J0,
+                                               int J1) {
+  double p0 = 0.0, p1 = 0.0;
+  int J = J0;
+  for (; J + 8 <= J1; J += 8) {
+    float4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = *reinterpret_cast<const float4 *>(row + (size_t)(J + u) * 256);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double *xj = x + 16 * (J + u);
+      double &p = (u & 1) ? p1 : p0;
+      p += (double)v[u].x * xj[0];
+      p += (double)v[u].y * xj[1];
+      p += (double)v[u].z * xj[2];
+      p += (double)v[u].w * xj[3];
+    }
+  }
+  for (; J + 4 <= J1; J += 4) {
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      v[u] = *reinterpret_cast<const float4 *>(row + (size_t)(J + u) * 256);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double *xj = x + 16 * (J + u);
+      double &p = (u & 1) ? p1 : p0;
+      p += (double)v[u].x * xj[0];
+      p += (double)v[u].y * xj[1];
+      p += (double)v[u].z * xj[2];
+      p += (double)v[u].w * xj[3];
+    }
+  }
+  for (; J < J1; ++J) {
+    const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)J * 256);
+    const double *xj = x + 16 * J;
+    p0 += (double)v.x * xj[0];
+    p0 += (double)v.y * xj[1];
+    p0 += (double)v.z * xj[2];
+    p0 += (double)v.w * xj[3];
+  }
+  return p0 + p1;
+}
+
+__global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
+  constexpr unsigned LO = 0xffffu;
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, q = lane >> 4;
+  double *xs = reinterpret_cast<double *>(smem + (size_t)wave * a.wave_bytes);
+  const int n1 = a.n1, n2 = a.n2, MT = a.MT;
+  double *xd = xs + n1;
+  double *tb = xd + 16 * MT;
+  const long nwv = (long)gridDim.x * (blockDim.x >> 6);
+  for (long d = (long)blockIdx.x * (blockDim.x >> 6) + wave; d < a.n_draws; d += nwv) {
+    if (!a.sol_ok[d])
+      continue;
+    const float *LU = a.sol + (size_t)d * a.sol_stride;
+    const float *sl = LU + (size_t)MT * MT * 256;
+    const double *th = a.theta + (size_t)d * a.P;
+    const double *rhs = a.rhs + (size_t)d * a.m;
+    double *y = a.y + (size_t)d * a.m;
+    for (int pass = 0; pass < (a.refine ? 2 : 1); ++pass) {
+      auto bval = [&](int node) -> double {
+        if (pass == 0)
+          return rhs[node];
+        double acc = rhs[node] - y[node];
+        for (int e = a.g_rowptr[node]; e < a.g_rowptr[node + 1]; ++e)
+          acc += th[a.g_idx[e]] * y[a.g_col[e]];
+        return acc;
+      };
+      /* forward through the prefix, wave by wave */
+      for (int w = 0; w < a.waves; ++w) {
+        for (int p = a.wave_q[w] + lane; p < a.wave_q[w + 1]; p += 64) {
+          double acc = bval(a.node_of[p]);
+          for (int e = a.fwd_ptr[p]; e < a.fwd_ptr[p + 1]; ++e) {
+            const unsigned c = a.fwd[e];
+            acc -= (double)sl[c & LO] * xs[c >> 16];
+          }
+          xs[p] = acc;
+        }
+        wave_order();
+      }
+      /* right-hand side of the dense part (identity padding: 0) */
+      for (int s2 = lane; s2 < 16 * MT; s2 += 64) {
+        double acc = 0.0;
+        if (s2 < n2) {
+          acc = bval(a.node_of[n1 + s2]);
+          for (int e = a.fwd_ptr[n1 + s2]; e < a.fwd_ptr[n1 + s2 + 1]; ++e) {
+            const unsigned c = a.fwd[e];
+            acc -= (double)sl[c & LO] * xs[c >> 16];
+          }
+        }
+        xd[s2] = acc;
+      }
+      wave_order();
+      /* dense forward: block rows K = 0 .. MT - 1 */
+      for (int K = 0; K < MT; ++K) {
+        const float *row = LU + (size_t)K * MT * 256 + r * 16 + 4 * q;
+        double part = tile_row_dot(row, xd + 4 * q, 0, K);
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        const double t = xd[16 * K + r] - part;
+        if (q == 0)
+          tb[r] = t;
+        wave_order();
+        /* y_K = L_KK^-1 t: unit diagonal, strictly lower part of the tile */
+        const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)K * 256);
+        double s = 0.0;
+        if (4 * q < r)
+          s += (double)v.x * tb[4 * q];
+        if (4 * q + 1 < r)
+          s += (double)v.y * tb[4 * q + 1];
+        if (4 * q + 2 < r)
+          s += (double)v.z * tb[4 * q + 2];
+        if (4 * q + 3 < r)
+          s += (double)v.w * tb[4 * q + 3];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (q == 0)
+          xd[16 * K + r] = t + s;
+        wave_order();
+      }
+      /* dense backward: block rows K = MT - 1 .. 0 */
+      for (int K = MT - 1; K >= 0; --K) {
+        const float *row = LU + (size_t)K * MT * 256 + r * 16 + 4 * q;
+        double part = tile_row_dot(row, xd + 4 * q, K + 1, MT);
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        const double t = xd[16 * K + r] - part;
+        if (q == 0)
+          tb[r] = t;
+        wave_order();
+        /* x_K = U_KK^-1 t: the tile's part on and above the diagonal */
+        const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)K * 256);
+        double s = 0.0;
+        if (4 * q >= r)
+          s += (double)v.x * tb[4 * q];
+        if (4 * q + 1 >= r)
+          s += (double)v.y * tb[4 * q + 1];
+        if (4 * q + 2 >= r)
+          s += (double)v.z * tb[4 * q + 2];
+        if (4 * q + 3 >= r)
+          s += (double)v.w * tb[4 * q + 3];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (q == 0)
+          xd[16 * K + r] = s;
+        wave_order();
+      }
+      /* backward through the prefix, waves in reverse order */
+      for (int w = a.waves - 1; w >= 0; --w) {
+        for (int p = a.wave_q[w] + lane; p < a.wave_q[w + 1]; p += 64) {
+          double acc = xs[p];
+          for (int e = a.bwd_ptr[p]; e < a.bwd_ptr[p + 1]; ++e) {
+            const unsigned c = a.bwd[e];
+            acc -= (double)sl[c & LO] * xs[c >> 16];
+          }
+          xs[p] = acc / (double)sl[a.piv[p]];
+        }
+        wave_order();
+      }
+      for (int i = lane; i < n1 + n2; i += 64) {
+        const int node = a.node_of[i];
+        y[node] = (pass == 0) ? xs[i] : y[node] + xs[i];
+      }
+      /* y of every lane before the residual reads it */
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+    }
   }
 }
 
@@ -620,6 +910,21 @@ template <int WR, int WC, int RT, int CT, int MODE, int MTT> RsKern rs_k() {
   return k;
 }
 
+/* The register kernel in solve mode (FP32 factors written out). */
+template <int WR, int WC, int RT, int CT, int MTT> const void *rs_solve_k() {
+  return (const void *)k_lu_rs<WR, WC, RT, CT, RS_FP32, MTT, false, true>;
+}
+const void *rs_solve_kern(int MT) {
+  const RsShape q = rs_shape(MT);
+  if (MT > kRsMaxTiles || MT < 1)
+    return nullptr;
+  if (q.RT == 6)
+    return rs_solve_k<2, 4, 6, 3, 0>();
+  if (q.RT == 8)
+    return rs_solve_k<2, 4, 8, 4, 0>();
+  return rs_solve_k<2, 4, 9, 5, 18>();
+}
+
 RsKern rs_kern(int MT, bool bf) {
   const RsShape q = rs_shape(MT);
   if (MT > kRsMaxTiles || MT < 1)
@@ -647,6 +952,83 @@ int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *t
   if (hipLaunchKernel(k.one, dim3(plan.rs_grid), dim3(k.threads), args, plan.rs_lds, st) !=
       hipSuccess)
     return SGDLM_ERR_DEVICE;
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
+size_t rs_solve_stride(const Structure &S, int MT) {
+  const size_t MTp = (size_t)rs_padded_tiles(MT);
+  return (MTp * MTp * 256 + (size_t)S.e_ns + 63) & ~(size_t)63;
+}
+
+int lu_subst_shape(const Structure &S, int MT, int n_cu, int *grid, size_t *lds) {
+  const int MTp = rs_padded_tiles(MT);
+  const size_t wave = (((size_t)S.e_n1 + 16 * (size_t)MTp + 16) * sizeof(double) + 15) &
+                      ~(size_t)15;
+  *lds = 4 * wave;
+  int per_cu = 0;
+  if (allow_dynamic_lds((const void *)k_lu_subst, *lds) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_lu_subst, 256,
+                                                   *lds) != hipSuccess ||
+      per_cu < 1) {
+    (void)hipGetLastError();
+    return SGDLM_ERR_ARG;
+  }
+  *grid = std::max(1, n_cu) * per_cu;
+  return SGDLM_OK;
+}
+
+int launch_lu_rs_factor(const Structure &S, const LuPlan &plan, long n, const double *theta,
+                        float *sol, int *sol_ok, int *flags, hipStream_t st) {
+  RsArgs a = rs_args(S, plan, n, theta, nullptr, flags);
+  a.sol = sol;
+  a.sol_stride = (long)plan.rs_sol_stride;
+  a.sol_ok = sol_ok;
+  const void *k = rs_solve_kern(a.MT);
+  if (!k)
+    return SGDLM_ERR_ARG;
+  if (allow_dynamic_lds(k, plan.rs_lds) != hipSuccess)
+    return SGDLM_ERR_DEVICE;
+  const RsShape q = rs_shape(a.MT);
+  const RsArgs *tab = nullptr;
+  GrpMap gm;
+  void *args[] = {&a, &tab, &gm};
+  const int grid = (int)std::max(1L, std::min<long>(plan.rs_grid, n));
+  if (hipLaunchKernel(k, dim3(grid), dim3(64 * q.WR * q.WC), args, plan.rs_lds, st) != hipSuccess)
+    return SGDLM_ERR_DEVICE;
+  return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
+}
+
+int launch_lu_subst(const Structure &S, const LuPlan &plan, long n, const double *theta,
+                    const double *rhs, double *y, const float *sol, const int *sol_ok,
+                    int refine, hipStream_t st) {
+  SubstArgs a{};
+  a.n_draws = n;
+  a.m = S.m;
+  a.MT = rs_padded_tiles(plan.rs_MT);
+  a.P = S.P;
+  a.theta = theta;
+  a.rhs = rhs;
+  a.y = y;
+  a.sol = sol;
+  a.sol_stride = (long)plan.rs_sol_stride;
+  a.sol_ok = sol_ok;
+  a.refine = refine;
+  a.n1 = S.e_n1;
+  a.n2 = S.e_n2;
+  a.waves = S.e_waves;
+  a.wave_q = S.e_wave_q;
+  a.node_of = S.e_node_of;
+  a.fwd_ptr = S.e_fwd_ptr;
+  a.bwd_ptr = S.e_bwd_ptr;
+  a.piv = S.e_tab + S.e_o_piv;
+  a.fwd = S.e_fwd;
+  a.bwd = S.e_bwd;
+  a.g_rowptr = S.g_rowptr;
+  a.g_col = S.g_col;
+  a.g_idx = S.g_idx;
+  a.wave_bytes = (int)(plan.subst_lds / 4);
+  const int grid = (int)std::max(1L, std::min<long>(plan.subst_grid, (n + 3) / 4));
+  hipLaunchKernelGGL(k_lu_subst, dim3(grid), dim3(256), plan.subst_lds, st, a);
   return hipGetLastError() == hipSuccess ? SGDLM_OK : SGDLM_ERR_DEVICE;
 }
 

@@ -223,6 +223,7 @@ struct sgdlm_ctx {
   int max_front_used = 0;       /* front limit of the current plan        */
   int lu_threads = 0, lu_nb = 0; /* fast LU kernel tuning (0 = automatic) */
   int lu_kind = SGDLM_LU_AUTO;  /* pinned LU kernel kind                     */
+  int lu_solve_kind = SGDLM_LU_AUTO; /* pinned forecast-solve kernel kind      */
   int max_front_default = 0;    /* front limit the default rule chose        */
   bool autotune = true;
   uint64_t tuned_key = 0;       /* structure key of `tuned` (0: none)        */
@@ -415,7 +416,7 @@ int ensure_sim_buffers(sgdlm_ctx *c) {
       (rc = dalloc(c, c->sim, "logdet", N * 8)) ||
       (rc = dalloc(c, c->sim, "w", N * 8)) ||
       (rc = dalloc(c, c->sim, "stats", 8 * 8)) ||
-      (rc = dalloc(c, c->sim, "lu_flags", (N + 1) * 4)))
+      (rc = dalloc(c, c->sim, "lu_flags", (N + 2) * 4)))
     return rc;
   const int chunks = vb_chunks((long)N);
   if ((rc = dalloc(c, c->sim, "part1", (size_t)chunks * m * 8)) ||
@@ -757,7 +758,8 @@ int vb_lu_plan(sgdlm_ctx *c, LuPlan *plan) {
 }
 
 int lu_workspace(sgdlm_ctx *c, int prec, int solve, LuPlan *plan) {
-  int rc = solve ? lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, plan) : vb_lu_plan(c, plan);
+  int rc = solve ? lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, plan, c->lu_solve_kind)
+                 : vb_lu_plan(c, plan);
   if (rc)
     return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
   return dalloc(c, c->sim, "lu_work", plan->work_bytes);
@@ -2603,19 +2605,45 @@ int sgdlm_get_lu_solve_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *
     return SGDLM_ERR_STATE;
   const int prec = (c->prec == SGDLM_PREC_FP64) ? SGDLM_PREC_FP64 : SGDLM_PREC_FP32;
   LuPlan plan;
-  int rc = lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, &plan);
+  int rc = lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, &plan, c->lu_solve_kind);
   if (rc)
     return fail(c, rc, "no LU solve configuration for m=%zu", c->m);
+  const bool reg = plan.rs_chunk > 0;
   if (grid)
-    *grid = plan.nb ? plan.fast_grid : plan.pivot_grid;
+    *grid = reg ? plan.rs_grid : plan.nb ? plan.fast_grid : plan.pivot_grid;
   if (lds)
-    *lds = plan.nb ? plan.lds : plan.pivot_lds;
+    *lds = reg ? plan.rs_lds : plan.nb ? plan.lds : plan.pivot_lds;
   if (nb)
-    *nb = plan.nb;
+    *nb = reg ? 16 : plan.nb;
   if (threads)
-    *threads = plan.nb ? plan.threads : 0;
+    *threads = reg ? 512 : plan.nb ? plan.threads : 0;
   if (slots_in_workspace)
-    *slots_in_workspace = plan.slot_stride != 0;
+    *slots_in_workspace = reg || plan.slot_stride != 0;
+  return SGDLM_OK;
+}
+
+int sgdlm_get_lu_solve_kernel(const sgdlm_ctx *cc, int *kind) {
+  sgdlm_ctx *c = const_cast<sgdlm_ctx *>(cc);
+  if (!c || !kind)
+    return SGDLM_ERR_ARG;
+  if (!c->parents_set)
+    return SGDLM_ERR_STATE;
+  const int prec = (c->prec == SGDLM_PREC_FP64) ? SGDLM_PREC_FP64 : SGDLM_PREC_FP32;
+  LuPlan plan;
+  int rc = lu_plan(c->S, prec, 1, c->n_cu, c->lds_max, 0, 0, &plan, c->lu_solve_kind);
+  if (rc)
+    return fail(c, rc, "no LU solve configuration for m=%zu", c->m);
+  *kind = plan.rs_chunk > 0 ? SGDLM_LU_REGISTER : plan.nb ? SGDLM_LU_BLOCKED : SGDLM_LU_PIVOTING;
+  return SGDLM_OK;
+}
+
+int sgdlm_set_lu_solve_kernel(sgdlm_ctx *c, int kind) {
+  if (!c)
+    return SGDLM_ERR_ARG;
+  if (kind != SGDLM_LU_AUTO && kind != SGDLM_LU_BLOCKED && kind != SGDLM_LU_REGISTER &&
+      kind != SGDLM_LU_PIVOTING)
+    return fail(c, SGDLM_ERR_ARG, "unknown LU solve kernel kind %d", kind);
+  c->lu_solve_kind = kind;
   return SGDLM_OK;
 }
 

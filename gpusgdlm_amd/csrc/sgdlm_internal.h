@@ -181,6 +181,12 @@ struct LuPlan {
    * fast kernel: rs_MT > 0 */
   int rs_MT = 0, rs_grid = 0;
   size_t rs_lds = 0, rs_panel_base = 0, rs_flag_base = 0;
+  /* forecast solves with the register kernel: draws in chunks of
+   * rs_chunk, each draw's factors (rs_sol_stride floats) in the workspace
+   * from byte rs_sol_off, then one flag word per chunk draw; the
+   * substitution kernel's launch shape */
+  int rs_chunk = 0, subst_grid = 0;
+  size_t rs_sol_stride = 0, rs_sol_off = 0, subst_lds = 0;
 };
 /* Largest dense Schur-complement size (multiple of 16) for which the
  * 64-wide fast LU fits: FP32 in half a CU's LDS, FP64 in a whole CU's. */
@@ -263,7 +269,17 @@ int lu_grp_wave(const Structure &S, const LuPlan &plan, long N, const double *th
                 double *logdet, LuGrpKernel *k);
 int lu_grp_wsch(const Structure &S, const LuPlan &plan, long N, const double *theta,
                 double *logdet, int *flags, LuGrpKernel *k);
-/* flags: N + 1 ints of scratch, as for launch_lu_logdet. */
+/* Forecast solves with the register kernel (k_lu_rs.hip): the factors of
+ * a chunk of draws written out (rs_solve_stride floats per draw), then one
+ * wavefront per draw substitutes (k_lu_subst). */
+size_t rs_solve_stride(const Structure &S, int MT);
+int lu_subst_shape(const Structure &S, int MT, int n_cu, int *grid, size_t *lds);
+int launch_lu_rs_factor(const Structure &S, const LuPlan &plan, long n, const double *theta,
+                        float *sol, int *sol_ok, int *flags, hipStream_t st);
+int launch_lu_subst(const Structure &S, const LuPlan &plan, long n, const double *theta,
+                    const double *rhs, double *y, const float *sol, const int *sol_ok,
+                    int refine, hipStream_t st);
+/* flags: N + 2 ints of scratch (count, list, and the total over chunks). */
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
                     const double *theta, void *work, const double *rhs,
                     double *y, int refine, int *flags, hipStream_t st);

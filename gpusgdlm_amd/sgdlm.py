@@ -392,6 +392,19 @@ class SGDLM:
         inv = {v: k for k, v in self.LU_KINDS.items()}
         self._call("sgdlm_set_lu_kernel", C.c_int(inv[kind]))
 
+    def lu_solve_kernel(self):
+        """Kernel kind of the forecast solves ("register", "blocked" or
+        "pivoting")."""
+        k = C.c_int()
+        self._call("sgdlm_get_lu_solve_kernel", C.byref(k))
+        return self.LU_KINDS[k.value]
+
+    def set_lu_solve_kernel(self, kind: str = "auto"):
+        """Pin the forecast-solve kernel kind ("auto", "blocked", "register",
+        "pivoting")."""
+        inv = {v: k for k, v in self.LU_KINDS.items()}
+        self._call("sgdlm_set_lu_solve_kernel", C.c_int(inv[kind]))
+
     def set_graph_mode(self, on: bool = True):
         """Capture the device part of a VB call as a hipGraph once and replay
         it on later calls (bit-identical results)."""

@@ -237,6 +237,14 @@ int sgdlm_get_lu_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
  * shared with the dense panel). */
 int sgdlm_get_lu_solve_config(const sgdlm_ctx *ctx, int *grid, size_t *lds_bytes,
                               int *nb, int *threads, int *slots_in_workspace);
+/* The forecast solves' kernel kind: SGDLM_LU_REGISTER (FP32 factors of
+ * dense blocks of 96..288 rows in registers, written out per chunk of draws
+ * and substituted by one wavefront per draw; the default where it fits),
+ * SGDLM_LU_BLOCKED (LDS panels, substitutions in the factoring workgroup) or
+ * SGDLM_LU_PIVOTING (every draw with row interchanges); pinning one
+ * (SGDLM_LU_AUTO: the default).  No counterpart in the reference. */
+int sgdlm_get_lu_solve_kernel(const sgdlm_ctx *ctx, int *kind);
+int sgdlm_set_lu_solve_kernel(sgdlm_ctx *ctx, int kind);
 /* Pins the fast VB LU kernel's workgroup size (256: two workgroups per CU,
  * 512: one) and panel width (16, 32, 64); 0 = automatic.  A configuration
  * that does not fit makes the next VB call fail with SGDLM_ERR_ARG. */

@@ -392,13 +392,19 @@ def test_logdet_lu_tuning_configs(hiplib, prec, cfgs, tol):
         assert np.max(np.abs(lw - ld)) <= tol
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-8)])
-def test_forecast_m400_fast_path(hiplib, prec, tol):
+@pytest.mark.parametrize("prec,tol,kind", [("fp64", 1e-9, "auto"), ("fp32", 1e-8, "blocked"),
+                                           ("fp32", 1e-8, "register")])
+def test_forecast_m400_fast_path(hiplib, prec, tol, kind):
     """Forecast solves at m = 400 through the sparse prefix and the
-    no-interchange dense LU (FP32: one FP64-residual refinement) match the
-    oracle's partial-pivoting solves."""
+    no-interchange dense LU (FP32: one FP64-residual refinement; the
+    blocked LU with its in-kernel substitutions, or the register-resident
+    LU writing its factors for the substitution kernel) match the oracle's
+    partial-pivoting solves."""
     mod = random_model(400, 20, T=60, seed=13)
     g, o = make_pair(mod, prec)
+    g.set_lu_solve_kernel(kind)
+    if kind != "auto":
+        assert g.lu_solve_kernel() == kind
     filter_steps(g, mod, 1, 50)
     filter_steps(o, mod, 1, 50)
     N = 256
@@ -419,8 +425,8 @@ def _dense_I_minus_Gamma(mod, th):
     return A
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_forecast_m400_pivoting_fallback(hiplib, prec):
+@pytest.mark.parametrize("prec,kind", [("fp64", "auto"), ("fp32", "blocked"), ("fp32", "register")])
+def test_forecast_m400_pivoting_fallback(hiplib, prec, kind):
     """Forecast solves whose draws need row interchanges: a seventh of the
     simultaneous coefficients of an m = 400 model get prior means near
     +-2.5, so most draws leave the no-interchange fast path (multiplier
@@ -431,9 +437,12 @@ def test_forecast_m400_pivoting_fallback(hiplib, prec):
     <= 1e-15 kappa (FP64, backward-stable LU) / (kappa u32)^2 with
     u32 = 6e-8 (FP32: one refinement step squares the unrefined error
     ~kappa u32; a missing refinement fails it for every kappa < 1e7).  The
-    lambda / theta draws themselves are bit-exact."""
+    lambda / theta draws themselves are bit-exact.  FP32 runs the blocked
+    LU and the register-resident one; both send the same draws (a
+    multiplier above 2) to the partial-pivoting kernel."""
     mod = random_model(400, 20, T=60, seed=13)
     g, o = make_pair(mod, prec)
+    g.set_lu_solve_kernel(kind)
     filter_steps(g, mod, 1, 50)
     filter_steps(o, mod, 1, 50)
     g.computePosterior(mod.y[50], mod.F(50))
@@ -451,8 +460,10 @@ def test_forecast_m400_pivoting_fallback(hiplib, prec):
     cfg = g.lu_solve_config()
     print(f"forecast fallback m=400 {prec}: solve plan {cfg}", flush=True)
     assert cfg["nb"] > 0
-    if prec == "fp32":
+    if kind == "blocked":
         assert cfg["threads"] == 256 and cfg["nb"] == 64 and cfg["slots_in_workspace"]
+    if kind == "register":
+        assert g.lu_solve_kernel() == "register" and cfg["threads"] == 512
     N = 512
     rg = g.computeForecastDebug(N, N, mod.F_forecast(51))
     fb = g.lu_fallback_count()
