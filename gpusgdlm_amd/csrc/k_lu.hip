@@ -58,6 +58,8 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <utility>
+#include <type_traits>
 #include <map>
 #include <math.h>
 #include <mutex>
@@ -500,6 +502,166 @@ __device__ void prefix_dense_solve(const T *A, int M, char *smem, const double *
       y[node] = (pass == 0) ? xs[i] : y[node] + xs[i];
     }
     __syncthreads();
+  }
+}
+
+/* f(std::integral_constant<int, i>) for i = 0 .. N - 1, each a separate
+ * inlined body (constant register indices however large the loop). */
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+/* Value of v in lane l, for every lane (an SGPR). */
+template <typename T> __device__ __forceinline__ T bcast_lane(T v, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+}
+
+/* Partial-pivoting LU of the tall panel Pn (nr <= 64 RS rows, jb columns,
+ * LDS row stride PS) by one wavefront, no workgroup barrier: row lane + 64 s
+ * in registers.  Rows are not exchanged while factoring: the pivot of
+ * column j is the largest |a_rj| over the rows not chosen yet (the
+ * candidates after LAPACK's exchanges; lowest row on ties), broadcast by
+ * readlane, then eliminated from the remaining rows with the workgroup
+ * loop's element operations (l = a_rj / u_jj, a_rc = fma(-l, u_jc, a_rc)).
+ * At the end pivot row j goes to position j, the rows it displaced from
+ * positions below jb to the positions the pivots left, and rowmap[K + .]
+ * is permuted the same way.  perm: jb ints of LDS scratch. */
+template <typename T, int NB, int PS, int RS>
+__device__ void panel_gepp_wave(T *Pn, int *rowmap, int *perm, int K, int nr, int jb, int lane,
+                                double &logdet, int &singular) {
+  T v[RS][NB];
+  bool live[RS];
+#pragma unroll
+  for (int s = 0; s < RS; ++s) {
+    const int r = lane + 64 * s;
+    live[s] = r < nr;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      v[s][c] = live[s] ? Pn[r * PS + c] : (T)0;
+  }
+  static_for<NB>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < jb) {
+      /* |a| ordered with NaN above "no row" and below every number */
+      T bv = (T)-2;
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int s = 0; s < RS; ++s) {
+        T x = fabs(v[s][j]);
+        if (x != x)
+          x = (T)-1;
+        if (live[s] && x > bv) {
+          bv = x;
+          bi = lane + 64 * s;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const T ov = __shfl_xor(bv, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      const int p = __builtin_amdgcn_readfirstlane(bi);
+      const int lp = p & 63, sp = p >> 6;
+      T u[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        u[c] = (T)0;
+#pragma unroll
+      for (int s = 0; s < RS; ++s)
+        if (sp == s) {
+#pragma unroll
+          for (int c = j; c < NB; ++c)
+            u[c] = bcast_lane(v[s][c], lp);
+        }
+#pragma unroll
+      for (int s = 0; s < RS; ++s)
+        if (s == sp && lane == lp)
+          live[s] = false;
+      T dj = u[j];
+      if (!(fabs(dj) > (T)0)) {
+        singular = 1;
+        dj = (T)1;
+      } else if (lane == 0) {
+        logdet += log(fabs((double)dj));
+      }
+      if (lane == 0)
+        perm[j] = p;
+#pragma unroll
+      for (int s = 0; s < RS; ++s) {
+        if (live[s]) {
+          const T l = v[s][j] / dj;
+          v[s][j] = l;
+#pragma unroll
+          for (int c = j + 1; c < NB; ++c)
+            if (c < jb)
+              v[s][c] = fma(-l, u[c], v[s][c]);
+        }
+      }
+    }
+  });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  /* rows below jb chosen as pivots (jb <= 32) */
+  unsigned pm = 0;
+  for (int j = 0; j < jb; ++j) {
+    const int pj = perm[j];
+    if (pj < jb)
+      pm |= 1u << pj;
+  }
+  int np[RS], old[RS];
+#pragma unroll
+  for (int s = 0; s < RS; ++s) {
+    const int r = lane + 64 * s;
+    np[s] = r;
+    old[s] = 0;
+    if (r < nr) {
+      old[s] = rowmap[K + r];
+      if (!live[s]) {
+        for (int j = 0; j < jb; ++j)
+          if (perm[j] == r)
+            np[s] = j;
+      } else if (r < jb) {
+        /* the t-th displaced row takes the t-th position a pivot left */
+        const int t = __popc(~pm & ((1u << r) - 1u));
+        int cnt = 0;
+        for (int j = 0; j < jb; ++j) {
+          const int pj = perm[j];
+          if (pj >= jb) {
+            if (cnt == t)
+              np[s] = pj;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int s = 0; s < RS; ++s) {
+    if (lane + 64 * s < nr) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        Pn[np[s] * PS + c] = v[s][c];
+      rowmap[K + np[s]] = old[s];
+    }
   }
 }
 
@@ -999,7 +1161,8 @@ __global__ __launch_bounds__(NT, 2) void k_lu_fast(LuArgs a0, const LuArgs *__re
  * Partial-pivoting path (fallback for flagged draws, forecast solves).
  * ====================================================================== */
 template <typename T, int NB, int MODE, bool SOLVE, bool GRP>
-__global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a0, const LuArgs *__restrict__ tab,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_lu_pivot(
+    LuArgs a0, const LuArgs *__restrict__ tab,
                                                   GrpMap gm) {
   typedef typename Vec4<T>::type V4;
   constexpr int WG = 256;
@@ -1088,6 +1251,18 @@ __global__ __launch_bounds__(256) void k_lu_pivot(LuArgs a0, const LuArgs *__res
       }
       __syncthreads();
 
+      /* FP32 panels up to 32 wide and 320 rows: one wavefront, rows in
+       * registers, no workgroup barrier per column */
+      if (sizeof(T) == 4 && NB <= 32 && nr <= 320) {
+        if (wave == 0) {
+          panel_gepp_wave<T, NB, PS, 5>(Pn, rowmap, reinterpret_cast<int *>(Linv), K, nr, jb,
+                                        lane, logdet, singular);
+          if (lane == 0)
+            redi[0] = singular;
+        }
+        __syncthreads();
+        singular = redi[0];
+      } else
       for (int j = 0; j < jb; ++j) {
         T bv = (T)-1;
         int bi = 0x7fffffff;
@@ -1666,8 +1841,7 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   if (!solve && (want_threads || want_nb || want_kind == SGDLM_LU_BLOCKED) && !plan->nb)
     return SGDLM_ERR_ARG;
   /* partial pivoting: two workgroups per CU when a 32- or 16-wide panel
-   * fits in half a CU (its per-column pivot searches are latency-bound),
-   * else one with the widest panel that fits */
+   * fits in half a CU, else one with the widest panel that fits */
   plan->pivot_grid = plan->grid;
   /* log-determinants with a sparse prefix: the pivoting kernel re-eliminates
    * the prefix in LDS (aliasing its panel region; the prefix flag after both)
