@@ -1900,6 +1900,27 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     plan->nb = 0;
   if (want_kind == SGDLM_LU_REGISTER && !plan->rs_MT)
     return SGDLM_ERR_ARG;
+  /* the register kernel with partial pivoting (FP32 log-determinants): its
+   * row-exchange buffer (32 rows of the trailing column tiles) after the
+   * staged panel tiles */
+  plan->rs_piv = 0;
+  plan->rs_swap_base = 0;
+  if (want_kind == SGDLM_LU_REGISTER_PIVOT) {
+    if (solve || prec != SGDLM_PREC_FP32 || S.e_rb_rows <= 0)
+      return SGDLM_ERR_ARG;
+    const int MT = S.e_rb_tiles;
+    const size_t swap_base = (S.e_rb_panel_base + rs_panel_bytes(MT) + 15) & ~(size_t)15;
+    const size_t need = swap_base + (size_t)32 * (MT - 1) * 16 * sizeof(float);
+    if (need > lds_max)
+      return SGDLM_ERR_ARG;
+    plan->rs_MT = MT;
+    plan->rs_grid = plan->grid;
+    plan->rs_lds = std::max(S.e_rb_lds, need);
+    plan->rs_panel_base = S.e_rb_panel_base;
+    plan->rs_flag_base = S.e_rb_flag_base;
+    plan->rs_piv = 1;
+    plan->rs_swap_base = swap_base;
+  }
   /* small models: one wavefront per draw on the whole matrix */
   plan->wave_M = 0;
   plan->wave_grid = 0;
@@ -1974,7 +1995,7 @@ int lu_plan_kind(const LuPlan &plan) {
   if (plan.wave_M > 0)
     return SGDLM_LU_WAVE;
   if (plan.rs_MT > 0)
-    return SGDLM_LU_REGISTER;
+    return plan.rs_piv ? SGDLM_LU_REGISTER_PIVOT : SGDLM_LU_REGISTER;
   return plan.nb ? SGDLM_LU_BLOCKED : SGDLM_LU_PIVOTING;
 }
 

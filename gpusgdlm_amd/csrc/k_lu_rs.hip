@@ -40,6 +40,9 @@
 
 #include <math.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "lu_prefix.h"
 #include "sgdlm_internal.h"
 
@@ -79,6 +82,9 @@ struct RsArgs {
   float *sol;
   long sol_stride;
   int *sol_ok;
+  /* PIV: LDS byte offset of the row-exchange buffer (32 rows x the
+   * trailing column tiles) */
+  int swap_base;
 };
 
 __device__ __forceinline__ f32x4 mfma_f32(float a, float b, f32x4 c) {
@@ -237,7 +243,188 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
   }
 }
 
-template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool GRP, bool SOLVE = false>
+typedef __attribute__((address_space(3))) int lds_i;
+
+/* f(std::integral_constant<int, i>) for i = 0 .. N - 1, each body inlined
+ * separately (constant register indices however long the loop). */
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+/* Largest (value, lowest index on ties) over the wavefront, by DPP: xor 1
+ * and 2 within quads, the half-row and row mirrors, then rows 0 / 2 into
+ * rows 1 / 3 and row 1 into rows 2 / 3; lane 63 ends with the result. */
+__device__ __forceinline__ int wave_argmax(float bv, int bi) {
+  auto step = [&](auto ctrl_c, auto rmask_c) {
+    constexpr int CTRL = decltype(ctrl_c)::value, RMASK = decltype(rmask_c)::value;
+    const float ov = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(-3.f), __float_as_int(bv), CTRL, RMASK, 0xF, false));
+    const int oi = __builtin_amdgcn_update_dpp(0x7fffffff, bi, CTRL, RMASK, 0xF, false);
+    if (ov > bv || (ov == bv && oi < bi)) {
+      bv = ov;
+      bi = oi;
+    }
+  };
+  step(std::integral_constant<int, 0xB1>{}, std::integral_constant<int, 0xF>{});  /* quad_perm 1,0,3,2 */
+  step(std::integral_constant<int, 0x4E>{}, std::integral_constant<int, 0xF>{});  /* quad_perm 2,3,0,1 */
+  step(std::integral_constant<int, 0x141>{}, std::integral_constant<int, 0xF>{}); /* row_half_mirror  */
+  step(std::integral_constant<int, 0x140>{}, std::integral_constant<int, 0xF>{}); /* row_mirror       */
+  step(std::integral_constant<int, 0x142>{}, std::integral_constant<int, 0xA>{}); /* row_bcast:15     */
+  step(std::integral_constant<int, 0x143>{}, std::integral_constant<int, 0xC>{}); /* row_bcast:31     */
+  return __builtin_amdgcn_readlane(bi, 63);
+}
+
+/* Partial-pivoting LU of the panel P (tile column k of S from row 16 k:
+ * nrows <= 320 rows of 16 columns, row stride TR) by one wavefront, rows
+ * r = lane + 64 s read and updated in LDS.  Rows are not exchanged while
+ * factoring: the pivot of column j is the largest |a_rj| over the rows not
+ * chosen yet (lowest row on ties, NaN below every number), its row read by
+ * every lane into scalar registers, and l = a_rj (1 / u_jj) stored in
+ * place of a_rj.  Then the rows go to pivot order: pivot j to row j, the
+ * rows it displaced from rows below 16 to the rows the pivots left.  The
+ * same exchanges, as (destination, source) pairs of panel rows after their
+ * count in pairs[0], are applied to the trailing columns by every
+ * wavefront.  Also writes |u_jj| to piv[j] and L_kk^-1 (unit lower,
+ * row-major) to Linv; a zero or NaN pivot sets bad.  tmp: 32 x 16 floats
+ * of scratch. */
+__device__ __attribute__((noinline)) void factor_panel_piv(lds_f *P, int nrows, lds_f *tmp,
+                                                          lds_i *pairs, lds_f *piv, lds_f *Linv,
+                                                          int lane, int &bad) {
+  constexpr int RS = 5;
+  unsigned live = 0;
+#pragma unroll
+  for (int s = 0; s < RS; ++s)
+    if (lane + 64 * s < nrows)
+      live |= 1u << s;
+  int pv[16];
+  static_for<16>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    constexpr int jq = j / 4, je = j % 4;
+    float bv = -2.f;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      if (64 * s < nrows) {
+        float x = fabsf(P[min(lane + 64 * s, nrows - 1) * TR + j]);
+        if (x != x)
+          x = -1.f;
+        if (((live >> s) & 1u) && x > bv) {
+          bv = x;
+          bi = lane + 64 * s;
+        }
+      }
+    }
+    const int p = __builtin_amdgcn_readfirstlane(wave_argmax(bv, bi));
+    pv[j] = p;
+    /* the pivot row, uniform: scalar registers */
+    float u[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const f32x4 t = lds_load4(P + p * TR + 4 * q);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        u[4 * q + e] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(t[e])));
+    }
+    float dj = u[j];
+    if (!(fabsf(dj) > 0.f)) {
+      bad = 1;
+      dj = 1.f;
+    }
+    if (lane == (p & 63))
+      live &= ~(1u << (p >> 6));
+    const float rdj = 1.f / dj;
+    /* every lane computes each slot in use (rows past the panel read a
+     * clamped row); only the stores of rows still live are predicated */
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      if (64 * s < nrows) {
+        const bool lv = (live >> s) & 1u;
+        lds_f *row = P + min(lane + 64 * s, nrows - 1) * TR;
+        f32x4 t[4] = {};
+#pragma unroll
+        for (int q = jq; q < 4; ++q)
+          t[q] = lds_load4(row + 4 * q);
+        const float l = t[jq][je] * rdj;
+        t[jq][je] = l;
+#pragma unroll
+        for (int c = j + 1; c < 16; ++c)
+          t[c / 4][c % 4] = fmaf(-l, u[c], t[c / 4][c % 4]);
+        if (lv) {
+#pragma unroll
+          for (int q = jq; q < 4; ++q)
+            lds_store4(row + 4 * q, t[q]);
+        }
+      }
+    }
+    wave_lds_sync();
+  });
+  /* the exchanges: pivot j to row j; each displaced row below 16 to a row
+   * a pivot left (both in increasing order) */
+  if (lane == 0) {
+    unsigned pm = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (pv[j] < 16)
+        pm |= 1u << pv[j];
+    int n = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (pv[j] != j) {
+        pairs[1 + 2 * n] = j;
+        pairs[2 + 2 * n] = pv[j];
+        ++n;
+      }
+    int r = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (pv[j] >= 16) {
+        while ((pm >> r) & 1u)
+          ++r;
+        pairs[1 + 2 * n] = pv[j];
+        pairs[2 + 2 * n] = r;
+        ++n;
+        ++r;
+      }
+    pairs[0] = n;
+  }
+  wave_lds_sync();
+  const int n = __builtin_amdgcn_readfirstlane(pairs[0]);
+  for (int e = lane; e < 16 * n; e += 64)
+    tmp[e] = P[pairs[2 + 2 * (e >> 4)] * TR + (e & 15)];
+  wave_lds_sync();
+  for (int e = lane; e < 16 * n; e += 64)
+    P[pairs[1 + 2 * (e >> 4)] * TR + (e & 15)] = tmp[e];
+  wave_lds_sync();
+  if (lane < 16) {
+    const int i = lane;
+    piv[i] = fabsf(P[i * TR + i]);
+    /* column i of L_kk^-1 by forward substitution */
+    float x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      float sum = 0.f;
+#pragma unroll
+      for (int q = 0; q <= r / 4; ++q) {
+        const f32x4 t = lds_load4(P + r * TR + 4 * q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (4 * q + e < r)
+            sum = fmaf(t[e], x[4 * q + e], sum);
+      }
+      x[r] = (r == i) ? 1.f : (r < i ? 0.f : -sum);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      Linv[r * TR + i] = x[r];
+  }
+}
+
+template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool GRP, bool SOLVE = false,
+          bool PIV = false>
 __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
     RsArgs a0, const RsArgs *__restrict__ tab, GrpMap gm) {
   constexpr int NW = WR * WC, NT = 64 * NW;
@@ -290,7 +477,7 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
       *flagB = 0;
     }
     prefix_eliminate<float>(sl, tb, th, a.pre, tid, NT, !a.tab_persist || d == (long)bid, bad,
-                            amax, ls, flagA, kMaxMultiplier);
+                            amax, ls, flagA, PIV ? kPrefixLooseMultiplier : kMaxMultiplier);
     bool stop = *flagA != 0;
     float *sol_d = nullptr;
     if constexpr (SOLVE) {
@@ -397,12 +584,69 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
             }
           }
         }
-        if (wr == k % WR) {
+        if (!PIV && wr == k % WR) {
           wave_lds_sync();
           factor_diag_tile((const lds_f *)(Xb + k * TSF), (lds_f *)LUb, (lds_f *)Urevb,
                            (lds_f *)Linvb, (lds_f *)UinvTb, (lds_f *)(pivb + 16 * k), lane, bad);
           if (bad)
             *flagA = 1;
+        }
+      }
+    };
+
+    /* PIV: the panel of column k (staged by its owners, all of them past a
+     * workgroup barrier) factored with partial pivoting by the owner of
+     * tile (k, k) */
+    lds_i *pairs = (lds_i *)LUb;
+    auto panel = [&](int k) {
+      if (wave == k % WR + WR * (k % WC)) {
+        factor_panel_piv((lds_f *)(Xb + k * TSF), 16 * (MT - k), (lds_f *)UinvTb, pairs,
+                         (lds_f *)(pivb + 16 * k), (lds_f *)Linvb, lane, bad);
+        if (bad)
+          *flagA = 1;
+      }
+    };
+
+    /* PIV: step k's row exchanges on the trailing columns J > k, through
+     * LDS: every source row out (WRITE), a workgroup barrier, every
+     * destination row in.  Row 16 k + r of tile I sits in the lanes with
+     * 4 g <= r % 16 < 4 g + 4, accumulator element r % 4. */
+    float *swapb = reinterpret_cast<float *>(smem + a.swap_base);
+    auto swap_rows = [&](int k, auto write_c) {
+      constexpr bool WRITE = decltype(write_c)::value;
+      const int np = __builtin_amdgcn_readfirstlane(pairs[0]);
+      const int nt = MT - k - 1;
+      for (int q = 0; q < np; ++q) {
+        const int row =
+            __builtin_amdgcn_readfirstlane(16 * k + pairs[WRITE ? 2 + 2 * q : 1 + 2 * q]);
+        const int I = row >> 4;
+        if (I % WR != wr)
+          continue;
+        const int ia = I / WR, lr = row & 15, rr = lr & 3;
+        const bool mine = g == (lr >> 2);
+        float *buf = swapb + q * nt * 16 + l16;
+#pragma unroll
+        for (int iac = 0; iac < RT; ++iac) {
+          if (iac == ia) {
+#pragma unroll
+            for (int jb = 0; jb < CT; ++jb) {
+              const int J = wc + WC * jb;
+              if (J > k && J < MT && mine) {
+                float *e = buf + (J - k - 1) * 16;
+                f32x4 c = acc[iac][jb];
+                if constexpr (WRITE) {
+                  *e = rr == 0 ? c[0] : rr == 1 ? c[1] : rr == 2 ? c[2] : c[3];
+                } else {
+                  const float v = *e;
+                  c[0] = rr == 0 ? v : c[0];
+                  c[1] = rr == 1 ? v : c[1];
+                  c[2] = rr == 2 ? v : c[2];
+                  c[3] = rr == 3 ? v : c[3];
+                  acc[iac][jb] = c;
+                }
+              }
+            }
+          }
         }
       }
     };
@@ -449,13 +693,23 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
       }
     };
 
-    if (!stop)
+    if (!stop) {
       phase1(0);
+      if constexpr (PIV) {
+        __syncthreads();
+        panel(0);
+      }
+    }
     for (int k = 0; k < MT && !stop; ++k) {
       step_sync<SOLVE>();
       if (*flagA) {
         stop = true;
         break;
+      }
+      if constexpr (PIV) {
+        swap_rows(k, std::true_type{});
+        __syncthreads();
+        swap_rows(k, std::false_type{});
       }
       /* ---- phase 3: U_kJ by the owners of row k, L_Ik by the others */
       if (wr == k % WR) {
@@ -499,6 +753,17 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
                 }
               }
             }
+          }
+        }
+      } else if constexpr (PIV) {
+        /* -L_Ik from the factored panel */
+        const int rank = ((wr - k % WR + WR) % WR - 1) * WC + wc;
+        for (int t = rank; k + 1 + t < MT; t += (WR - 1) * WC) {
+          const int I = k + 1 + t;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int o = I * TSF + (4 * g + r) * TR + l16;
+            Lb[o] = -Xb[o];
           }
         }
       } else {
@@ -548,6 +813,10 @@ __global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
       if (k + 1 < MT) {
         update(k, [&](int J) { return J == k + 1; });
         phase1(k + 1);
+        if constexpr (PIV) {
+          __syncthreads();
+          panel(k + 1);
+        }
         update(k, [&](int J) { return J > k + 1; });
       }
       if constexpr (SOLVE) {
@@ -892,6 +1161,7 @@ RsArgs rs_args(const Structure &S, const LuPlan &plan, long N, const double *the
   a.sg_idx = S.e_rb_sg_idx;
   a.st_pairs = S.e_rb_pairs;
   a.zero_pair = S.e_zero_pair;
+  a.swap_base = (int)plan.rs_swap_base;
   return a;
 }
 
@@ -902,10 +1172,10 @@ struct RsKern {
   int threads = 0;
 };
 
-template <int WR, int WC, int RT, int CT, int MODE, int MTT> RsKern rs_k() {
+template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool PIV = false> RsKern rs_k() {
   RsKern k;
-  k.one = (const void *)k_lu_rs<WR, WC, RT, CT, MODE, MTT, false>;
-  k.grp = (const void *)k_lu_rs<WR, WC, RT, CT, MODE, MTT, true>;
+  k.one = (const void *)k_lu_rs<WR, WC, RT, CT, MODE, MTT, false, false, PIV>;
+  k.grp = (const void *)k_lu_rs<WR, WC, RT, CT, MODE, MTT, true, false, PIV>;
   k.threads = 64 * WR * WC;
   return k;
 }
@@ -925,10 +1195,18 @@ const void *rs_solve_kern(int MT) {
   return rs_solve_k<2, 4, 9, 5, 18>();
 }
 
-RsKern rs_kern(int MT, bool bf) {
+/* piv: partial pivoting inside the kernel (FP32 only) */
+RsKern rs_kern(int MT, bool bf, bool piv) {
   const RsShape q = rs_shape(MT);
-  if (MT > kRsMaxTiles || MT < 1)
+  if (MT > kRsMaxTiles || MT < 1 || (piv && bf))
     return RsKern();
+  if (piv) {
+    if (q.RT == 6)
+      return rs_k<2, 4, 6, 3, RS_FP32, 0, true>();
+    if (q.RT == 8)
+      return rs_k<2, 4, 8, 4, RS_FP32, 0, true>();
+    return rs_k<2, 4, 9, 5, RS_FP32, 18, true>();
+  }
   if (q.RT == 6)
     return bf ? rs_k<2, 4, 6, 3, RS_BF16X3, 0>() : rs_k<2, 4, 6, 3, RS_FP32, 0>();
   if (q.RT == 8)
@@ -941,7 +1219,7 @@ RsKern rs_kern(int MT, bool bf) {
 int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
                  double *logdet, int *flags, hipStream_t st) {
   RsArgs a = rs_args(S, plan, N, theta, logdet, flags);
-  const RsKern k = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3);
+  const RsKern k = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3, plan.rs_piv != 0);
   if (!k.one)
     return SGDLM_ERR_ARG;
   if (allow_dynamic_lds(k.one, plan.rs_lds) != hipSuccess)
@@ -1035,7 +1313,7 @@ int launch_lu_subst(const Structure &S, const LuPlan &plan, long n, const double
 int lu_grp_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
               double *logdet, int *flags, LuGrpKernel *k) {
   const RsArgs a = rs_args(S, plan, N, theta, logdet, flags);
-  const RsKern rk = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3);
+  const RsKern rk = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3, plan.rs_piv != 0);
   if (!rk.grp)
     return SGDLM_ERR_ARG;
   k->fn = rk.grp;
@@ -1046,7 +1324,7 @@ int lu_grp_rs(const Structure &S, const LuPlan &plan, long N, const double *thet
   k->units = N;
   /* ms per draw on one workgroup (size model; 0.15 ms at 18 tiles) */
   const double r = a.MT / 18.0;
-  k->cost = (double)N * (0.15 * r * r + 5e-7 * (double)S.e_fma);
+  k->cost = (double)N * ((plan.rs_piv ? 0.19 : 0.15) * r * r + 5e-7 * (double)S.e_fma);
   k->args.assign(reinterpret_cast<const unsigned char *>(&a),
                  reinterpret_cast<const unsigned char *>(&a) + sizeof(a));
   k->work_field = -1;

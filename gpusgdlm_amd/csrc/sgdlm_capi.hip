@@ -796,6 +796,10 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
   std::vector<Cand> cands;
   if (c->prec != SGDLM_PREC_FP64)
     cands.push_back({SGDLM_LU_REGISTER, 0, 0});
+  /* the register kernel with partial pivoting: wins where most draws need
+   * row interchanges (early in a series) */
+  if (c->prec == SGDLM_PREC_FP32)
+    cands.push_back({SGDLM_LU_REGISTER_PIVOT, 0, 0});
   for (int th : {256, 512})
     for (int nb : {64, 32, 16})
       cands.push_back({SGDLM_LU_BLOCKED, th, nb});
@@ -992,7 +996,8 @@ std::vector<uintptr_t> vb_signature(sgdlm_ctx *c, long N, const VbDev &D, const 
           (uintptr_t)p.rs_MT, (uintptr_t)p.rs_grid, (uintptr_t)p.rs_lds, (uintptr_t)p.pivot_nb,
           (uintptr_t)p.pivot_grid, (uintptr_t)p.pivot_lds, (uintptr_t)p.pivot_schur_M,
           (uintptr_t)p.work_bytes, (uintptr_t)p.wave_M, (uintptr_t)p.wave_grid,
-          (uintptr_t)p.wsch_M, (uintptr_t)p.wsch_grid, (uintptr_t)p.wsch_lds};
+          (uintptr_t)p.wsch_M, (uintptr_t)p.wsch_grid, (uintptr_t)p.wsch_lds,
+          (uintptr_t)p.rs_piv, (uintptr_t)p.rs_swap_base};
 }
 
 /* Graph mode: the VB step of run_vb captured once as a hipGraph (re-captured
@@ -2520,7 +2525,7 @@ int sgdlm_get_lu_kernel(const sgdlm_ctx *cc, int *kind, int *dense_dim, int *piv
 int sgdlm_set_lu_kernel(sgdlm_ctx *c, int kind) {
   if (!c)
     return SGDLM_ERR_ARG;
-  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_WAVE_PREFIX)
+  if (kind < SGDLM_LU_AUTO || kind > SGDLM_LU_REGISTER_PIVOT)
     return fail(c, SGDLM_ERR_ARG, "unknown LU kernel kind %d", kind);
   c->lu_kind = kind;
   return SGDLM_OK;

@@ -181,6 +181,11 @@ struct LuPlan {
    * fast kernel: rs_MT > 0 */
   int rs_MT = 0, rs_grid = 0;
   size_t rs_lds = 0, rs_panel_base = 0, rs_flag_base = 0;
+  /* the same kernel with partial pivoting (SGDLM_LU_REGISTER_PIVOT, FP32
+   * log-determinants): rs_piv = 1, row-exchange buffer at LDS byte
+   * rs_swap_base */
+  int rs_piv = 0;
+  size_t rs_swap_base = 0;
   /* forecast solves with the register kernel: draws in chunks of
    * rs_chunk, each draw's factors (rs_sol_stride floats) in the workspace
    * from byte rs_sol_off, then one flag word per chunk draw; the

@@ -364,7 +364,8 @@ class SGDLM:
         self._call("sgdlm_get_lu_fallback_count", C.byref(n))
         return n.value
 
-    LU_KINDS = {0: "auto", 1: "blocked", 2: "register", 3: "pivoting", 4: "wave", 5: "wave_prefix"}
+    LU_KINDS = {0: "auto", 1: "blocked", 2: "register", 3: "pivoting", 4: "wave", 5: "wave_prefix",
+                6: "register_pivoting"}
 
     def lu_config(self):
         """Launch shape of the VB LU kernel for the current structure and
@@ -388,7 +389,7 @@ class SGDLM:
 
     def set_lu_kernel(self, kind: str = "auto"):
         """Pin the VB LU kernel kind ("auto", "blocked", "register", "pivoting",
-        "wave", "wave_prefix")."""
+        "wave", "wave_prefix", "register_pivoting")."""
         inv = {v: k for k, v in self.LU_KINDS.items()}
         self._call("sgdlm_set_lu_kernel", C.c_int(inv[kind]))
 

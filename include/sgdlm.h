@@ -60,6 +60,11 @@ extern "C" {
                                per draw, the prefix on its own LDS slots, the
                                Schur complement in registers, partial
                                pivoting                                     */
+#define SGDLM_LU_REGISTER_PIVOT 6 /* dense block resident in one CU's
+                               registers (FP32, up to 288 rows) with partial
+                               pivoting: each 16-wide column panel factored
+                               by one wavefront from LDS, its row exchanges
+                               applied to the register tiles                */
 
 typedef struct sgdlm_ctx sgdlm_ctx;
 
