@@ -430,12 +430,12 @@ __global__ void k_grp_flags(const VbModel *__restrict__ tab, int n) {
     tab[i].lu_flags[0] = tab[i].flag_init;
 }
 
-/* After it: the weight statistics of every model side by side (one copy
- * to the host). */
+/* After it: the weight statistics of every model side by side, each
+ * followed by its LU fallback count (one copy to the host). */
 __global__ void k_grp_stats(const VbModel *__restrict__ tab, int n, double *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 4 * n)
-    out[i] = tab[i / 4].stats[i % 4];
+  if (i < 5 * n)
+    out[i] = i % 5 < 4 ? tab[i / 5].stats[i % 5] : (double)tab[i / 5].lu_flags[0];
 }
 
 } // namespace
@@ -528,7 +528,7 @@ int launch_moments_grp(const VbModel *tab, const int *ser_model, int n_series, i
 }
 
 void launch_grp_stats(const VbModel *tab, int n, double *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_grp_stats, dim3((4 * n + 255) / 256), dim3(256), 0, st, tab, n, out);
+  hipLaunchKernelGGL(k_grp_stats, dim3((5 * n + 255) / 256), dim3(256), 0, st, tab, n, out);
 }
 
 } // namespace sgdlm

@@ -91,6 +91,11 @@ struct LuTuned {
   double draw_wg_ms = 0.0;
   int candidates = 0;
   bool from_cache = false;
+  /* the faster of the kinds that pivot every draw (partial-pivoting kernel,
+   * register kernel with interchanges) and its time: what a call runs when
+   * the last call handed nearly every draw to the fallback */
+  int pa_kind = SGDLM_LU_PIVOTING;
+  double pa_ms = 0.0;
 };
 
 /* A captured VB step and what it was captured for: replayed only while
@@ -138,7 +143,8 @@ struct VbGroup {
   uint32_t *d_epochs = nullptr;
   double *d_stats = nullptr;
   uint32_t *h_epochs = nullptr; /* pinned */
-  double *h_stats = nullptr;    /* pinned, 4 per model */
+  double *h_stats = nullptr;    /* pinned, 5 per model (4 weight
+                                 * statistics, the LU fallback count) */
   std::vector<GrpLaunch> lu1, lu2; /* longest first */
   /* the LU launches of a stage run concurrently on these streams, forked
    * from and joined back to the leader's stream through the events */
@@ -246,6 +252,12 @@ struct sgdlm_ctx {
   std::map<std::string, DBuf> sim;
   bool vb_valid = false;
   bool lu_flags_valid = false; /* lu_flags holds the last LU call's count */
+  /* the adaptive LU of run_vb (kind not pinned): draws of the last call that
+   * ran the no-interchange kernel and how many of them it flagged; calls
+   * since it; whether the current call pivots every draw */
+  long fb_n = 0, run_n = 0;
+  int fb_count = 0, pa_streak = 0;
+  bool ran_pa = false;
 
   bool timing = false;
   std::map<std::string, std::pair<double, long>> times;
@@ -816,7 +828,9 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
     return fail(c, SGDLM_ERR_DEVICE, "hipEventCreate failed");
   }
   int rc = SGDLM_OK, n = 0;
-  double best_ms = 1e300;
+  double best_ms = 1e300, pa_ms = 1e300;
+  int pa_kind = SGDLM_LU_PIVOTING;
+  bool have_np = false; /* a kind that is not interchanges-for-every-draw */
   for (int mf : fronts) {
     if (mf != c->max_front_used && (rc = build_elimination(c, c->ents, mf)))
       break;
@@ -849,6 +863,21 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
       if (rc)
         break;
       ++n;
+      /* the kinds that pivot every draw are kept apart: timed on early
+       * draws they win, and a structure keeps its tuned kind for the rest
+       * of the series; a call that finds nearly every draw flagged runs
+       * them instead (run_vb) */
+      const bool pa = k.kind == SGDLM_LU_PIVOTING || k.kind == SGDLM_LU_REGISTER_PIVOT;
+      if (pa && ms < pa_ms) {
+        pa_ms = ms;
+        pa_kind = k.kind;
+      }
+      if (pa && have_np)
+        continue;
+      if (!pa && !have_np) {
+        have_np = true;
+        best_ms = 1e30f;
+      }
       if (ms < best_ms) {
         best_ms = ms;
         const long wgs = p.wave_M > 0   ? std::min<long>(p.wave_grid, (nt + 3) / 4)
@@ -875,6 +904,8 @@ int tune_vb_lu(sgdlm_ctx *c, long N, VbDev &D, LuTuned *best) {
   if (n == 0)
     return fail(c, SGDLM_ERR_ARG, "no LU configuration fits m=%zu in LDS", c->m);
   best->best_ms = best_ms;
+  best->pa_kind = pa_kind;
+  best->pa_ms = pa_ms < 1e299 ? pa_ms : 0.0;
   best->candidates = n;
   best->from_cache = false;
   best->tune_ms =
@@ -961,6 +992,9 @@ int enqueue_vb(sgdlm_ctx *c, long N, VbDev &D, LuPlan &plan, bool t_on, bool tun
   }
   CHECK_LAUNCH(c);
   HIPCHK(c, hipMemcpyAsync(c->h_stats, D.B.stats, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  /* the LU's fallback count (draws handed to the partial-pivoting kernel) */
+  HIPCHK(c, hipMemcpyAsync(c->h_stats + 4, D.lu_flags, sizeof(int), hipMemcpyDeviceToHost,
                            c->stream));
   return SGDLM_OK;
 }
@@ -1051,6 +1085,32 @@ bool tuning_pending(sgdlm_ctx *c) {
          !(c->tuned_key == structure_key(c) && c->tuned.max_front == c->max_front_used);
 }
 
+/* Adaptive LU of a VB call (kind not pinned, structure tuned): after a call
+ * whose no-interchange kernel flagged more than 90 % of the draws (early in
+ * a series) the call pivots every draw with the tuned kind for that,
+ * trying the no-interchange kernel again every 8th call.  Sets c->ran_pa;
+ * the counts are taken by vb_status. */
+void adaptive_vb_plan(sgdlm_ctx *c, LuPlan *plan) {
+  c->ran_pa = false;
+  c->run_n = 0;
+  const int pk = lu_plan_kind(*plan);
+  if (lu_pinned(c) || tuning_pending(c) || (pk != SGDLM_LU_BLOCKED && pk != SGDLM_LU_REGISTER))
+    return;
+  const bool high = c->fb_n > 0 && (double)c->fb_count > 0.9 * (double)c->fb_n;
+  if (!high) {
+    c->pa_streak = 0;
+    return;
+  }
+  if (++c->pa_streak % 8 == 0)
+    return;
+  LuPlan pa;
+  if (!lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, 0, 0, &pa, c->tuned.pa_kind) ||
+      !lu_plan(c->S, c->prec, 0, c->n_cu, c->lds_max, 0, 0, &pa, SGDLM_LU_PIVOTING)) {
+    *plan = pa;
+    c->ran_pa = true;
+  }
+}
+
 /* VB on the context's simulation pool (results stay on the device). */
 int run_vb(sgdlm_ctx *c) {
   int rc;
@@ -1064,8 +1124,12 @@ int run_vb(sgdlm_ctx *c) {
   D.theta = SIM<double>(c, "theta");
   D.logdet = SIM<double>(c, "logdet");
   D.lu_flags = SIM<int>(c, "lu_flags");
+  adaptive_vb_plan(c, &plan);
+  if (c->ran_pa && (rc = dalloc(c, c->sim, "lu_work", plan.work_bytes)))
+    return rc;
   D.lu_work = c->sim["lu_work"].ptr;
   D.B = vb_buffers(c);
+  c->run_n = (long)c->nsim;
   if (c->graph_mode && !c->timing && !tuning_pending(c))
     rc = run_vb_graph(c, (long)c->nsim, D, plan);
   else
@@ -1081,6 +1145,11 @@ int run_vb(sgdlm_ctx *c) {
  * (each |det(I - Gamma_i)| zero or non-finite) is an error, also when the
  * results stay on the device. */
 int vb_status(sgdlm_ctx *c) {
+  if (c->run_n > 0 && !c->ran_pa) {
+    std::memcpy(&c->fb_count, c->h_stats + 4, sizeof(int));
+    c->fb_n = c->run_n;
+  }
+  c->run_n = 0;
   if (!(c->h_stats[1] > 0.0))
     return fail(c, SGDLM_ERR_NUMERIC,
                 "every importance weight is zero (all |det(I-Gamma)| zero or non-finite)");
@@ -1097,6 +1166,8 @@ int group_member(sgdlm_ctx *c, VbDev *D, LuPlan *plan) {
     return rc;
   if ((rc = vb_lu_plan(c, plan)))
     return fail(c, rc, "no LU configuration fits m=%zu (M=%d) in LDS", c->m, c->S.M);
+  adaptive_vb_plan(c, plan);
+  c->run_n = (long)c->nsim;
   D->lam = SIM<double>(c, "lam");
   D->theta = SIM<double>(c, "theta");
   D->logdet = SIM<double>(c, "logdet");
@@ -1260,7 +1331,7 @@ int build_vb_group(sgdlm_ctx *const *cs, int n, const std::vector<VbDev> &Ds,
     return o;
   };
   const size_t o_tab = take(sizeof(VbModel) * n), o_ser = take(sizeof(int) * ser.size()),
-               o_ep = take(sizeof(uint32_t) * n), o_st = take(sizeof(double) * 4 * n);
+               o_ep = take(sizeof(uint32_t) * n), o_st = take(sizeof(double) * 5 * n);
   struct Pending {
     std::vector<unsigned char> args;
     std::vector<int> inst, base, count;
@@ -1317,7 +1388,7 @@ int build_vb_group(sgdlm_ctx *const *cs, int n, const std::vector<VbDev> &Ds,
   }
   if (hipHostMalloc((void **)&g->h_epochs, sizeof(uint32_t) * n, hipHostMallocDefault) !=
           hipSuccess ||
-      hipHostMalloc((void **)&g->h_stats, sizeof(double) * 4 * n, hipHostMallocDefault) !=
+      hipHostMalloc((void **)&g->h_stats, sizeof(double) * 5 * n, hipHostMallocDefault) !=
           hipSuccess) {
     (void)hipGetLastError();
     drop_vb_group(g);
@@ -1464,7 +1535,7 @@ int enqueue_vb_group(sgdlm_ctx *c, VbGroup *g, hipStream_t st, bool t_on) {
   }
   launch_grp_stats(g->d_tab, g->n, g->d_stats, st);
   CHECK_LAUNCH(c);
-  HIPCHK(c, hipMemcpyAsync(g->h_stats, g->d_stats, sizeof(double) * 4 * g->n,
+  HIPCHK(c, hipMemcpyAsync(g->h_stats, g->d_stats, sizeof(double) * 5 * g->n,
                            hipMemcpyDeviceToHost, st));
   return SGDLM_OK;
 }
@@ -1651,7 +1722,10 @@ int run_vb_group(sgdlm_ctx *const *cs, int n, std::vector<int> &rcs) {
   for (size_t q = 0; q < b->parts.size(); ++q) {
     for (size_t k = 0; k < b->members[q].size(); ++k) {
       sgdlm_ctx *c = cs[b->members[q][k]];
-      std::memcpy(c->h_stats, b->parts[q]->h_stats + 4 * k, 4 * sizeof(double));
+      const double *hs = b->parts[q]->h_stats + 5 * k;
+      std::memcpy(c->h_stats, hs, 4 * sizeof(double));
+      const int fb = (int)hs[4];
+      std::memcpy(c->h_stats + 4, &fb, sizeof(int));
       c->lu_flags_valid = true;
       c->vb_valid = true;
     }

@@ -223,3 +223,50 @@ def test_register_pivoting_kernel_vs_oracle(hiplib, m, k, T):
         kappa = np.linalg.cond(_dense(mod, th[:, :, i]))
         print(f"  draw {i}: kappa {kappa:.2e} err {err[i]:.2e}", flush=True)
         assert err[i] <= 6e-8 * kappa
+
+
+def test_adaptive_lu_early_then_late(hiplib):
+    """Kind not pinned, FP32, m = 400 early in a series (every draw needs
+    row interchanges): after the call that tunes the structure, calls pivot
+    every draw (the fallback count is the draw count) except every 8th,
+    which tries the no-interchange kernel again; log-dets stay within the
+    oracle's bars.  The tuned kind is not one that pivots every draw, so
+    late in the series, once a retry finds few flagged draws (within 8
+    calls), the no-interchange kernel runs again."""
+    mod = random_model(400, 20, T=252, seed=20250101)
+    N = 2048
+    g = SGDLM(strict=True)
+    g.set_rng_state(SEED, 0)
+    g.set_precision("fp32")
+    configure(g, mod)
+    filter_steps(g, mod, 1, 30)
+    g.computePosterior(mod.y[30], mod.F(30))
+    counts = []
+    for call in range(9):
+        g.set_rng_state(SEED, 0)
+        g.computeVBPosterior(N, N)
+        counts.append(g.lu_fallback_count())
+        if call == 1:
+            lw_early = g.get_vb_result()["log_weights"].copy()
+    t = g.lu_autotune()
+    print(f"  early: tuned {t}, fallback per call {counts}", flush=True)
+    assert t["kernel"] not in ("pivoting", "register_pivoting")
+    assert counts[0] > 0.9 * N
+    assert all(c == N for c in counts[1:8])
+    assert counts[8] > 0.9 * N and counts[8] < N or counts[8] == N
+    ld = _oracle_logdets(mod, N, t_post=30)
+    err = np.abs(lw_early - ld)
+    print(f"  early |err| median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert np.median(err) <= 5e-5 and np.sum(err > 5e-3) <= N // 20
+    g.computePrior()
+    filter_steps(g, mod, 31, 250)
+    g.computePosterior(mod.y[250], mod.F(250))
+    late = []
+    for call in range(9):
+        g.computeVBPosterior(N, N)
+        late.append(g.lu_fallback_count())
+    print(f"  late: fallback per call {late}", flush=True)
+    # within 8 calls the no-interchange kernel is tried again, finds few
+    # flagged draws, and the calls after it keep it
+    assert min(late) < N // 10 and late[-1] < N // 10
+    g.close()
