@@ -1878,7 +1878,8 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
   if ((solve ? prec == SGDLM_PREC_FP32 : prec != SGDLM_PREC_FP64) && !want_threads && !want_nb &&
       (want_kind == SGDLM_LU_AUTO || want_kind == SGDLM_LU_REGISTER) && S.e_rb_rows > 0) {
     plan->rs_MT = S.e_rb_tiles;
-    plan->rs_grid = plan->grid;
+    plan->rs_nw = S.e_rb_nw;
+    plan->rs_grid = plan->grid * (S.e_rb_nw == 4 ? 2 : 1);
     plan->rs_lds = S.e_rb_lds;
     plan->rs_panel_base = S.e_rb_panel_base;
     plan->rs_flag_base = S.e_rb_flag_base;
@@ -1911,10 +1912,11 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     const int MT = S.e_rb_tiles;
     const size_t swap_base = (S.e_rb_panel_base + rs_panel_bytes(MT) + 15) & ~(size_t)15;
     const size_t need = swap_base + (size_t)32 * (MT - 1) * 16 * sizeof(float);
-    if (need > lds_max)
+    if (need > lds_max / (S.e_rb_nw == 4 ? 2 : 1))
       return SGDLM_ERR_ARG;
     plan->rs_MT = MT;
-    plan->rs_grid = plan->grid;
+    plan->rs_nw = S.e_rb_nw;
+    plan->rs_grid = plan->grid * (S.e_rb_nw == 4 ? 2 : 1);
     plan->rs_lds = std::max(S.e_rb_lds, need);
     plan->rs_panel_base = S.e_rb_panel_base;
     plan->rs_flag_base = S.e_rb_flag_base;

@@ -425,7 +425,7 @@ __device__ __attribute__((noinline)) void factor_panel_piv(lds_f *P, int nrows, 
 
 template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool GRP, bool SOLVE = false,
           bool PIV = false>
-__global__ __launch_bounds__(64 * WR * WC, (WR * WC) / 4) void k_lu_rs(
+__global__ __launch_bounds__(64 * WR * WC, 2) void k_lu_rs(
     RsArgs a0, const RsArgs *__restrict__ tab, GrpMap gm) {
   constexpr int NW = WR * WC, NT = 64 * NW;
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -1103,7 +1103,11 @@ size_t rs_panel_bytes(int MT) { return (size_t)(3 * MT + 4) * TSF * sizeof(float
 struct RsShape {
   int WR, WC, RT, CT, MTT;
 };
-static RsShape rs_shape(int MT) {
+/* nw = 4: four wavefronts (2 x 2 grid, 6 x 6 tiles each) for blocks of up
+ * to 12 tiles, two workgroups per CU */
+static RsShape rs_shape(int MT, int nw = 8) {
+  if (nw == 4 && MT <= 12)
+    return {2, 2, 6, 6, 0};
   if (MT <= 12)
     return {2, 4, 6, 3, 0};
   if (MT <= 16)
@@ -1111,13 +1115,13 @@ static RsShape rs_shape(int MT) {
   return {2, 4, 9, 5, 18};
 }
 
-int rs_padded_tiles(int MT) {
-  const RsShape q = rs_shape(MT);
+int rs_padded_tiles(int MT, int nw) {
+  const RsShape q = rs_shape(MT, nw);
   return q.MTT > 0 ? q.MTT : MT;
 }
 
-int rs_stage_stride(int MT) {
-  const RsShape q = rs_shape(MT);
+int rs_stage_stride(int MT, int nw) {
+  const RsShape q = rs_shape(MT, nw);
   return 16 * q.WC * q.CT + 4;
 }
 
@@ -1135,9 +1139,9 @@ RsArgs rs_args(const Structure &S, const LuPlan &plan, long N, const double *the
   a.flag_list = flags + 1;
   a.pre = PrefixTabs{S.e_ns, S.e_waves, S.e_n1, S.e_tab_words, S.e_o_piv, S.e_o_mptr, S.e_o_mult,
                      S.e_o_tptr, S.e_o_tslot, S.e_o_cptr, S.e_o_contrib, S.e_slot_init, S.e_tab};
-  a.pre_base = 0;
+  a.pre_base = (int)S.e_rb_pre_base;
   a.slot_bytes = (int)((((size_t)S.e_ns * 4) + 15) & ~(size_t)15);
-  a.tab_persist = 1;
+  a.tab_persist = S.e_rb_tab_persist;
   a.flag_base = (int)plan.rs_flag_base;
   a.red_base = a.flag_base + 16;
   const int P0 = (int)plan.rs_panel_base;
@@ -1149,7 +1153,7 @@ RsArgs rs_args(const Structure &S, const LuPlan &plan, long N, const double *the
   a.uinvt_base = a.linv_base + TSF * 4;
   a.urev_base = a.uinvt_base + TSF * 4;
   a.piv_base = a.urev_base + TSF * 4;
-  a.stage_base = P0;
+  a.stage_base = (int)S.e_rb_stage_base;
   a.band_rows = S.e_rb_rows;
   a.n_band = S.e_rb_nband;
   a.blk_ptr = S.e_rb_blk_ptr;
@@ -1184,10 +1188,12 @@ template <int WR, int WC, int RT, int CT, int MODE, int MTT, bool PIV = false> R
 template <int WR, int WC, int RT, int CT, int MTT> const void *rs_solve_k() {
   return (const void *)k_lu_rs<WR, WC, RT, CT, RS_FP32, MTT, false, true>;
 }
-const void *rs_solve_kern(int MT) {
-  const RsShape q = rs_shape(MT);
+const void *rs_solve_kern(int MT, int nw) {
+  const RsShape q = rs_shape(MT, nw);
   if (MT > kRsMaxTiles || MT < 1)
     return nullptr;
+  if (q.WC == 2)
+    return rs_solve_k<2, 2, 6, 6, 0>();
   if (q.RT == 6)
     return rs_solve_k<2, 4, 6, 3, 0>();
   if (q.RT == 8)
@@ -1196,10 +1202,15 @@ const void *rs_solve_kern(int MT) {
 }
 
 /* piv: partial pivoting inside the kernel (FP32 only) */
-RsKern rs_kern(int MT, bool bf, bool piv) {
-  const RsShape q = rs_shape(MT);
+RsKern rs_kern(int MT, bool bf, bool piv, int nw) {
+  const RsShape q = rs_shape(MT, nw);
   if (MT > kRsMaxTiles || MT < 1 || (piv && bf))
     return RsKern();
+  if (q.WC == 2) {
+    if (piv)
+      return rs_k<2, 2, 6, 6, RS_FP32, 0, true>();
+    return bf ? rs_k<2, 2, 6, 6, RS_BF16X3, 0>() : rs_k<2, 2, 6, 6, RS_FP32, 0>();
+  }
   if (piv) {
     if (q.RT == 6)
       return rs_k<2, 4, 6, 3, RS_FP32, 0, true>();
@@ -1219,7 +1230,7 @@ RsKern rs_kern(int MT, bool bf, bool piv) {
 int launch_lu_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
                  double *logdet, int *flags, hipStream_t st) {
   RsArgs a = rs_args(S, plan, N, theta, logdet, flags);
-  const RsKern k = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3, plan.rs_piv != 0);
+  const RsKern k = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3, plan.rs_piv != 0, plan.rs_nw);
   if (!k.one)
     return SGDLM_ERR_ARG;
   if (allow_dynamic_lds(k.one, plan.rs_lds) != hipSuccess)
@@ -1261,12 +1272,12 @@ int launch_lu_rs_factor(const Structure &S, const LuPlan &plan, long n, const do
   a.sol = sol;
   a.sol_stride = (long)plan.rs_sol_stride;
   a.sol_ok = sol_ok;
-  const void *k = rs_solve_kern(a.MT);
+  const void *k = rs_solve_kern(a.MT, plan.rs_nw);
   if (!k)
     return SGDLM_ERR_ARG;
   if (allow_dynamic_lds(k, plan.rs_lds) != hipSuccess)
     return SGDLM_ERR_DEVICE;
-  const RsShape q = rs_shape(a.MT);
+  const RsShape q = rs_shape(a.MT, plan.rs_nw);
   const RsArgs *tab = nullptr;
   GrpMap gm;
   void *args[] = {&a, &tab, &gm};
@@ -1282,7 +1293,7 @@ int launch_lu_subst(const Structure &S, const LuPlan &plan, long n, const double
   SubstArgs a{};
   a.n_draws = n;
   a.m = S.m;
-  a.MT = rs_padded_tiles(plan.rs_MT);
+  a.MT = rs_padded_tiles(plan.rs_MT, plan.rs_nw);
   a.P = S.P;
   a.theta = theta;
   a.rhs = rhs;
@@ -1313,12 +1324,12 @@ int launch_lu_subst(const Structure &S, const LuPlan &plan, long n, const double
 int lu_grp_rs(const Structure &S, const LuPlan &plan, long N, const double *theta,
               double *logdet, int *flags, LuGrpKernel *k) {
   const RsArgs a = rs_args(S, plan, N, theta, logdet, flags);
-  const RsKern rk = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3, plan.rs_piv != 0);
+  const RsKern rk = rs_kern(a.MT, plan.prec == SGDLM_PREC_BF16X3, plan.rs_piv != 0, plan.rs_nw);
   if (!rk.grp)
     return SGDLM_ERR_ARG;
   k->fn = rk.grp;
   k->threads = rk.threads;
-  k->per_cu = 1;
+  k->per_cu = plan.rs_nw == 4 ? 2 : 1;
   k->lds = plan.rs_lds;
   k->ws_wg = 0;
   k->units = N;

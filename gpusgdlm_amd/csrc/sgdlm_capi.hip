@@ -517,9 +517,14 @@ int build_wave_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
 }
 
 /* LDS layout and banded Schur tables of the register-resident LU
- * (k_lu_rs.hip) for plan P: the prefix slots and tables, two flags and the
- * reduction words, then one region shared by the staged panel tiles and
- * the staging band of S (as many 16-row groups as fit).  Leaves
+ * (k_lu_rs.hip) for plan P.  Dense blocks of up to 12 tiles: four
+ * wavefronts, two workgroups per CU in half the LDS each: two flags and
+ * the reduction words, then the prefix slots and tables followed by the
+ * staging band of S, all aliased by the staged panel tiles of the dense
+ * LU (the slots are dead once S is gathered; the tables are loaded per
+ * draw).  Otherwise (or when that does not fit) eight wavefronts, one
+ * workgroup per CU: the prefix slots and tables, the flags, then one region
+ * shared by the staged panel tiles and the staging band.  Leaves
  * e_rb_rows = 0 when the dense block is outside 6..18 tiles or does not
  * fit. */
 int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
@@ -527,18 +532,42 @@ int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
   S.e_rb_rows = 0;
   if (P.M2 / 16 < 6 || P.M2 / 16 > kRsMaxTiles)
     return SGDLM_OK;
-  const int MT = rs_padded_tiles(P.M2 / 16);
   const size_t pre = (((size_t)P.ns * 4 + 15) & ~(size_t)15) + P.tab.size() * 4;
-  const size_t flag_base = (pre + 15) & ~(size_t)15;
-  const size_t panel_base = flag_base + 16 + 8 * sizeof(double);
-  const size_t panel = rs_panel_bytes(MT);
-  const int stride = rs_stage_stride(MT);
-  int BR = 0;
-  for (int br = 16; br <= 16 * MT; br += 16)
-    if (panel_base + std::max(panel, (size_t)br * stride * 4) <= c->lds_max)
-      BR = br;
-  if (BR == 0)
-    return SGDLM_OK;
+  const size_t pre_al = (pre + 15) & ~(size_t)15;
+  int nw = 8, MT = rs_padded_tiles(P.M2 / 16), stride = rs_stage_stride(MT), BR = 0;
+  int tab_persist = 1;
+  size_t flag_base = 0, panel_base = 0, pre_base = 0, stage_base = 0, lds = 0;
+  if (P.M2 / 16 <= 12) {
+    const int MT4 = rs_padded_tiles(P.M2 / 16, 4), st4 = rs_stage_stride(MT4, 4);
+    const size_t pb = 16 + 8 * sizeof(double) + 16, pnl = rs_panel_bytes(MT4);
+    for (int br = 16; br <= 16 * MT4; br += 16)
+      if (pb + std::max(pnl, pre_al + (size_t)br * st4 * 4) <= c->lds_max / 2)
+        BR = br;
+    if (BR) {
+      nw = 4;
+      MT = MT4;
+      stride = st4;
+      tab_persist = 0;
+      flag_base = 0;
+      panel_base = pb;
+      pre_base = pb;
+      stage_base = pb + pre_al;
+      lds = pb + std::max(pnl, pre_al + (size_t)BR * st4 * 4);
+    }
+  }
+  if (nw == 8) {
+    flag_base = pre_al;
+    panel_base = flag_base + 16 + 8 * sizeof(double);
+    const size_t panel = rs_panel_bytes(MT);
+    for (int br = 16; br <= 16 * MT; br += 16)
+      if (panel_base + std::max(panel, (size_t)br * stride * 4) <= c->lds_max)
+        BR = br;
+    if (BR == 0)
+      return SGDLM_OK;
+    pre_base = 0;
+    stage_base = panel_base;
+    lds = panel_base + std::max(panel, ((size_t)BR * stride * 4 + 15) & ~(size_t)15);
+  }
   BandPlan B;
   build_band_plan(P, BR, stride, 16 * MT, B);
   int rc;
@@ -560,9 +589,13 @@ int build_band_tables(sgdlm_ctx *c, const ElimPlan &P) {
   S.e_rb_nband = B.n_band;
   S.e_rb_flag_base = flag_base;
   S.e_rb_panel_base = panel_base;
-  S.e_rb_lds = panel_base + std::max(panel, ((size_t)BR * stride * 4 + 15) & ~(size_t)15);
+  S.e_rb_lds = lds;
   S.e_rb_rows = BR;
   S.e_rb_tiles = MT;
+  S.e_rb_nw = nw;
+  S.e_rb_tab_persist = tab_persist;
+  S.e_rb_pre_base = pre_base;
+  S.e_rb_stage_base = stage_base;
   return SGDLM_OK;
 }
 
@@ -1031,7 +1064,7 @@ std::vector<uintptr_t> vb_signature(sgdlm_ctx *c, long N, const VbDev &D, const 
           (uintptr_t)p.pivot_grid, (uintptr_t)p.pivot_lds, (uintptr_t)p.pivot_schur_M,
           (uintptr_t)p.work_bytes, (uintptr_t)p.wave_M, (uintptr_t)p.wave_grid,
           (uintptr_t)p.wsch_M, (uintptr_t)p.wsch_grid, (uintptr_t)p.wsch_lds,
-          (uintptr_t)p.rs_piv, (uintptr_t)p.rs_swap_base};
+          (uintptr_t)p.rs_piv, (uintptr_t)p.rs_swap_base, (uintptr_t)p.rs_nw};
 }
 
 /* Graph mode: the VB step of run_vb captured once as a hipGraph (re-captured
@@ -2569,7 +2602,7 @@ int sgdlm_get_lu_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *nb, in
   if (nb)
     *nb = (wave || wsch) ? 0 : reg ? 16 : plan.nb;
   if (threads)
-    *threads = (wave || wsch) ? 256 : reg ? 512 : (plan.nb ? plan.threads : 0);
+    *threads = (wave || wsch) ? 256 : reg ? 64 * plan.rs_nw : (plan.nb ? plan.threads : 0);
   return SGDLM_OK;
 }
 
@@ -2695,7 +2728,7 @@ int sgdlm_get_lu_solve_config(const sgdlm_ctx *cc, int *grid, size_t *lds, int *
   if (nb)
     *nb = reg ? 16 : plan.nb;
   if (threads)
-    *threads = reg ? 512 : plan.nb ? plan.threads : 0;
+    *threads = reg ? 64 * plan.rs_nw : plan.nb ? plan.threads : 0;
   if (slots_in_workspace)
     *slots_in_workspace = reg || plan.slot_stride != 0;
   return SGDLM_OK;

@@ -46,6 +46,12 @@ struct Structure {
    * kernel does not fit this structure. */
   int e_rb_rows = 0, e_rb_stride = 0, e_rb_nband = 0, e_rb_tiles = 0;
   size_t e_rb_lds = 0, e_rb_panel_base = 0, e_rb_flag_base = 0;
+  /* wavefronts of the register kernel's shape (8: one workgroup per CU; 4:
+   * blocks of up to 12 tiles, two workgroups per CU, the prefix slots,
+   * tables and staging band aliasing the panel region, the tables loaded
+   * per draw); LDS offsets of the prefix region and the staging band */
+  int e_rb_nw = 8, e_rb_tab_persist = 1;
+  size_t e_rb_pre_base = 0, e_rb_stage_base = 0;
   const int *e_rb_blk_ptr = nullptr, *e_rb_st_off = nullptr, *e_rb_brounds = nullptr,
             *e_rb_bcbase = nullptr, *e_rb_sg_ptr = nullptr, *e_rb_sg_off = nullptr,
             *e_rb_sg_idx = nullptr;
@@ -186,6 +192,9 @@ struct LuPlan {
    * rs_swap_base */
   int rs_piv = 0;
   size_t rs_swap_base = 0;
+  /* wavefronts per workgroup of the register kernel's shape (Structure
+   * e_rb_nw); 4: two workgroups per CU */
+  int rs_nw = 8;
   /* forecast solves with the register kernel: draws in chunks of
    * rs_chunk, each draw's factors (rs_sol_stride floats) in the workspace
    * from byte rs_sol_off, then one flag word per chunk draw; the
@@ -232,8 +241,8 @@ int lu_plan_cap(const Structure &S, size_t work_cap, LuPlan *plan);
 size_t rs_panel_bytes(int MT);
 /* row stride (floats) of the register-resident LU's LDS staging band, and
  * the tile count it pads a dense block of MT tiles to (identity) */
-int rs_stage_stride(int MT);
-int rs_padded_tiles(int MT);
+int rs_stage_stride(int MT, int nw = 8);
+int rs_padded_tiles(int MT, int nw = 8);
 constexpr int kRsMaxTiles = 18;
 /* Register-resident LU log-determinants (plan.rs_MT > 0); flagged draws are
  * appended to flags for the partial-pivoting kernel. */

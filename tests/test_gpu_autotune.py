@@ -270,3 +270,27 @@ def test_adaptive_lu_early_then_late(hiplib):
     # flagged draws, and the calls after it keep it
     assert min(late) < N // 10 and late[-1] < N // 10
     g.close()
+
+
+@pytest.mark.parametrize("m,k,prec,T", [(150, 14, "fp32", 40), (250, 20, "fp32", 40),
+                                        (200, 16, "bf16x3", 40), (250, 20, "fp32", 5)])
+def test_register_four_wave_shape_vs_oracle(hiplib, m, k, prec, T):
+    """Dense blocks of up to 12 tiles run the register kernel with four
+    wavefronts and two workgroups per CU (prefix slots, tables and staging
+    band aliasing the panel region): log-dets within the LU bars of the
+    oracle; early in a series (T = 5) the flagged draws come back from the
+    partial-pivoting kernel."""
+    mod = random_model(m, k, T=T + 2, seed=5 if m == 250 else 77 + m)
+    N = 512
+
+    def pin(g):
+        g.set_lu_autotune(False)
+        g.set_lu_kernel("register")
+    r = _gpu(mod, prec, N, pin, t_post=T)
+    print(f"register m={m} {prec} T={T}: {r['cfg']}", flush=True)
+    assert r["cfg"]["kernel"] == "register" and r["cfg"]["threads"] == 256
+    assert r["cfg"]["dense_dim"] <= 192
+    ld = _oracle_logdets(mod, N, t_post=T)
+    err = np.abs(r["lw"] - ld)
+    print(f"  |err| median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert np.median(err) <= 5e-5 and np.sum(~(err <= TOL[prec])) <= N // 50

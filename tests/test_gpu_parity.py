@@ -415,6 +415,28 @@ def test_forecast_m400_fast_path(hiplib, prec, tol, kind):
     assert relerr(yg, yo) <= tol
 
 
+@pytest.mark.parametrize("m,k", [(200, 16), (250, 20)])
+def test_forecast_register_four_wave_vs_oracle(hiplib, m, k):
+    """FP32 forecast solves with the register kernel's four-wavefront shape
+    (dense blocks of up to 12 tiles, two workgroups per CU): factors written
+    out, one wavefront per draw substitutes with one FP64-residual
+    refinement; the oracle's partial-pivoting solves within 1e-8."""
+    mod = random_model(m, k, T=52, seed=5 if m == 250 else 277)
+    g, o = make_pair(mod, "fp32")
+    g.set_lu_solve_kernel("register")
+    filter_steps(g, mod, 1, 50)
+    filter_steps(o, mod, 1, 50)
+    cfg = g.lu_solve_config()
+    print(f"forecast m={m} register solve {cfg}", flush=True)
+    assert g.lu_solve_kernel() == "register" and cfg["threads"] == 256
+    N = 512
+    yg = g.computeForecast(N, N, mod.F_forecast(50))
+    yo = o.computeForecast(N, N, mod.F_forecast(50))
+    assert g.lu_fallback_count() <= N // 8
+    assert relerr(yg, yo) <= 1e-8
+    g.close()
+
+
 def _dense_I_minus_Gamma(mod, th):
     A = np.eye(mod.m)
     spf = mod.sp_u32.ravel(order="F")
