@@ -89,9 +89,15 @@ __global__ __launch_bounds__(64) void k_cholesky(int m, int max_p, const unsigne
 
 /* k_cholesky over the series of several models (one workgroup per series
  * of every model; dynamic LDS for the largest max_p). */
+/* Also stores each model's LU fallback counter (flag_init) from the block
+ * of its first series: the first launch of a group step, so that no
+ * separate launch (which can wait for a free CU behind other parts' LU
+ * workgroups) is needed before the LU. */
 __global__ __launch_bounds__(64) void k_cholesky_grp(const VbModel *__restrict__ tab,
                                                      const int *__restrict__ ser_model) {
   const VbModel &md = tab[ser_model[blockIdx.x]];
+  if ((int)blockIdx.x == md.ser0 && threadIdx.x == 0)
+    md.lu_flags[0] = md.flag_init;
   cholesky_body((int)blockIdx.x - md.ser0, md.max_p, md.p, md.C, md.U);
 }
 

@@ -423,13 +423,6 @@ __global__ __launch_bounds__(64) void k_vb_final_grp(const VbModel *__restrict__
                 md.mean_Q, md.newton_iter, (int)blockIdx.x - md.ser0);
 }
 
-/* Before the LU of a step over several models: each model's flags[0]. */
-__global__ void k_grp_flags(const VbModel *__restrict__ tab, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    tab[i].lu_flags[0] = tab[i].flag_init;
-}
-
 /* After it: the weight statistics of every model side by side, each
  * followed by its LU fallback count (one copy to the host). */
 __global__ void k_grp_stats(const VbModel *__restrict__ tab, int n, double *out) {
@@ -488,10 +481,6 @@ void launch_moments(const Structure &S, long N, const double *lam,
                      chunks, S.p, m_t, n_t, s_t, B.part1, B.part2, B.mean_lambda,
                      B.mean_log_lambda, B.mean_m, B.mean_C, B.mean_n,
                      B.mean_s, B.mean_Q, B.newton_iter);
-}
-
-void launch_grp_flags(const VbModel *tab, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_grp_flags, dim3((n + 63) / 64), dim3(64), 0, st, tab, n);
 }
 
 void launch_weights_grp(const VbModel *tab, int n, hipStream_t st) {

@@ -1552,7 +1552,7 @@ int enqueue_vb_group(sgdlm_ctx *c, VbGroup *g, hipStream_t st, bool t_on) {
   CHECK_LAUNCH(c);
   {
     StageTimer t(c, "batch_lu", t_on);
-    launch_grp_flags(g->d_tab, g->n, st);
+    /* the fallback counters were set by the Cholesky launch */
     if ((rc = launch_grp_stage(c, g, g->lu1, st)) || (rc = launch_grp_stage(c, g, g->lu2, st)))
       return rc;
   }

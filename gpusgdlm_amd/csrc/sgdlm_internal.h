@@ -326,7 +326,6 @@ void launch_cholesky_grp(const VbModel *tab, const int *ser_model, int n_series,
                          hipStream_t st);
 int launch_draw_grp(const VbModel *tab, const int *ser_model, const uint32_t *epochs,
                     int n_series, long max_N, int max_p, hipStream_t st);
-void launch_grp_flags(const VbModel *tab, int n, hipStream_t st);
 void launch_weights_grp(const VbModel *tab, int n, hipStream_t st);
 int launch_moments_grp(const VbModel *tab, const int *ser_model, int n_series, int max_chunks,
                        int max_p, hipStream_t st);
