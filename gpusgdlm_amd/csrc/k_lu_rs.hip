@@ -962,8 +962,31 @@ __device__ __forceinline__ double tile_row_dot(const float *row, const double *x
   return p0 + p1;
 }
 
-__global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
+/* acc - sum over entries e0 <= e < e1 of lst (slot | x index << 16) of
+ * sl[slot] * xs[x index], in the order of e; the list and slot loads of
+ * eight entries are issued together (the lists are serial per lane) */
+__device__ __forceinline__ double sub_entries(double acc, const unsigned *__restrict__ lst,
+                                              int e0, int e1, const float *__restrict__ sl,
+                                              const double *xs) {
   constexpr unsigned LO = 0xffffu;
+  for (int e = e0; e < e1; e += 8) {
+    unsigned c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      c[u] = e + u < e1 ? lst[e + u] : 0u;
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = e + u < e1 ? sl[c[u] & LO] : 0.f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (e + u < e1)
+        acc -= (double)v[u] * xs[c[u] >> 16];
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -986,38 +1009,48 @@ __global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
         if (pass == 0)
           return rhs[node];
         double acc = rhs[node] - y[node];
-        for (int e = a.g_rowptr[node]; e < a.g_rowptr[node + 1]; ++e)
-          acc += th[a.g_idx[e]] * y[a.g_col[e]];
+        const int e1 = a.g_rowptr[node + 1];
+        for (int e = a.g_rowptr[node]; e < e1; e += 8) {
+          int gi[8], gc[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            gi[u] = e + u < e1 ? a.g_idx[e + u] : 0;
+            gc[u] = e + u < e1 ? a.g_col[e + u] : 0;
+          }
+          double tv[8], yv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            tv[u] = e + u < e1 ? th[gi[u]] : 0.0;
+            yv[u] = e + u < e1 ? y[gc[u]] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (e + u < e1)
+              acc += tv[u] * yv[u];
+        }
         return acc;
       };
       /* forward through the prefix, wave by wave */
       for (int w = 0; w < a.waves; ++w) {
         for (int p = a.wave_q[w] + lane; p < a.wave_q[w + 1]; p += 64) {
-          double acc = bval(a.node_of[p]);
-          for (int e = a.fwd_ptr[p]; e < a.fwd_ptr[p + 1]; ++e) {
-            const unsigned c = a.fwd[e];
-            acc -= (double)sl[c & LO] * xs[c >> 16];
-          }
-          xs[p] = acc;
+          xs[p] = sub_entries(bval(a.node_of[p]), a.fwd, a.fwd_ptr[p], a.fwd_ptr[p + 1], sl, xs);
         }
         wave_order();
       }
       /* right-hand side of the dense part (identity padding: 0) */
       for (int s2 = lane; s2 < 16 * MT; s2 += 64) {
         double acc = 0.0;
-        if (s2 < n2) {
-          acc = bval(a.node_of[n1 + s2]);
-          for (int e = a.fwd_ptr[n1 + s2]; e < a.fwd_ptr[n1 + s2 + 1]; ++e) {
-            const unsigned c = a.fwd[e];
-            acc -= (double)sl[c & LO] * xs[c >> 16];
-          }
-        }
+        if (s2 < n2)
+          acc = sub_entries(bval(a.node_of[n1 + s2]), a.fwd, a.fwd_ptr[n1 + s2],
+                            a.fwd_ptr[n1 + s2 + 1], sl, xs);
         xd[s2] = acc;
       }
       wave_order();
       /* dense forward: block rows K = 0 .. MT - 1 */
       for (int K = 0; K < MT; ++K) {
         const float *row = LU + (size_t)K * MT * 256 + r * 16 + 4 * q;
+        /* the diagonal tile's load issued with the row's */
+        const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)K * 256);
         double part = tile_row_dot(row, xd + 4 * q, 0, K);
         part += __shfl_xor(part, 16);
         part += __shfl_xor(part, 32);
@@ -1026,7 +1059,6 @@ __global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
           tb[r] = t;
         wave_order();
         /* y_K = L_KK^-1 t: unit diagonal, strictly lower part of the tile */
-        const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)K * 256);
         double s = 0.0;
         if (4 * q < r)
           s += (double)v.x * tb[4 * q];
@@ -1045,6 +1077,7 @@ __global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
       /* dense backward: block rows K = MT - 1 .. 0 */
       for (int K = MT - 1; K >= 0; --K) {
         const float *row = LU + (size_t)K * MT * 256 + r * 16 + 4 * q;
+        const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)K * 256);
         double part = tile_row_dot(row, xd + 4 * q, K + 1, MT);
         part += __shfl_xor(part, 16);
         part += __shfl_xor(part, 32);
@@ -1053,7 +1086,6 @@ __global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
           tb[r] = t;
         wave_order();
         /* x_K = U_KK^-1 t: the tile's part on and above the diagonal */
-        const float4 v = *reinterpret_cast<const float4 *>(row + (size_t)K * 256);
         double s = 0.0;
         if (4 * q >= r)
           s += (double)v.x * tb[4 * q];
@@ -1072,12 +1104,9 @@ __global__ __launch_bounds__(256) void k_lu_subst(SubstArgs a) {
       /* backward through the prefix, waves in reverse order */
       for (int w = a.waves - 1; w >= 0; --w) {
         for (int p = a.wave_q[w] + lane; p < a.wave_q[w + 1]; p += 64) {
-          double acc = xs[p];
-          for (int e = a.bwd_ptr[p]; e < a.bwd_ptr[p + 1]; ++e) {
-            const unsigned c = a.bwd[e];
-            acc -= (double)sl[c & LO] * xs[c >> 16];
-          }
-          xs[p] = acc / (double)sl[a.piv[p]];
+          const float pv = sl[a.piv[p]];
+          const double acc = sub_entries(xs[p], a.bwd, a.bwd_ptr[p], a.bwd_ptr[p + 1], sl, xs);
+          xs[p] = acc / (double)pv;
         }
         wave_order();
       }
