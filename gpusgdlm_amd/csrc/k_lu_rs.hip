@@ -152,14 +152,34 @@ __device__ __forceinline__ void lds_store4(lds_f *p, f32x4 v) {
   *reinterpret_cast<lds_f4 *>(p) = v;
 }
 
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+/* v from lane CTRL - 0x150 of each row of 16 lanes (row_newbcast), or the
+ * row mirrored (CTRL 0x140) */
+template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 /* LU without interchanges of the 16 x 16 tile X (LDS, row stride TR) and
  * its inverse factors: one wavefront.  Writes L^-1 row-major to Linv, U^-1
- * transposed to UinvT (UinvT[c][i] = U^-1[i][c]) and |u_ii| to piv[i]; LU
- * and Urev are scratch tiles.  A zero or NaN pivot, or a multiplier beyond
- * kMaxMultiplier sets bad. */
-__device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *Linv,
-                                 lds_f *UinvT, lds_f *piv, int lane, int &bad) {
-  const int i = lane & 15;
+ * transposed to UinvT (UinvT[c][i] = U^-1[i][c]) and |u_ii| to piv[i].  A
+ * zero or NaN pivot, or a multiplier beyond kMaxMultiplier sets bad.
+ * Every row of 16 lanes eliminates the tile (lane i holds row i; pivot
+ * rows by readlane).  The inverses come from the same registers: rows of
+ * the lower triangles L (unit) and T = J U J (row i = row 15 - i of U
+ * reversed, by a row mirror) are formed right-looking, x_i -= (T_ic /
+ * T_cc) x_c with row c broadcast within each row of lanes (DPP), then
+ * scaled by 1 / T_ii; T^-1 = J U^-1 J.  Lane rows 0 and 2 take L, 1 and 3
+ * take T, and rows 0 / 1 keep the even columns, 2 / 3 the odd ones. */
+__device__ void factor_diag_tile(const lds_f *X, lds_f *Linv, lds_f *UinvT, lds_f *piv,
+                                 int lane, int &bad) {
+  const int i = lane & 15, g = lane >> 4;
   float v[16];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -168,11 +188,9 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
     for (int e = 0; e < 4; ++e)
       v[4 * q + e] = t[e];
   }
-  /* one row per lane; pivot row j broadcast from lane j with readlane */
+  /* pivot row j broadcast from lane j with readlane */
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    /* the whole pivot row first: back-to-back readlanes cover each
-     * other's SGPR write-to-read latency */
     float p[16];
 #pragma unroll
     for (int c = j; c < 16; ++c)
@@ -190,56 +208,59 @@ __device__ void factor_diag_tile(const lds_f *X, lds_f *LU, lds_f *Urev, lds_f *
     for (int c = j + 1; c < 16; ++c)
       v[c] = fmaf(-l, p[c], v[c]);
   }
+  /* the lane's row index again, opaque to the compiler and available only
+   * after the elimination, so that the row masks below are formed where
+   * they are used instead of being shared with the elimination's (lane
+   * masks live across the function are scalar registers its callers must
+   * spill) */
+  int io = i;
+  asm volatile("" : "+v"(io) : "v"(v[15]));
+  const bool tk = (g & 1) != 0;
+  float a[16];
+  static_for<16>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
+    const float m = dpp_f<0x140>(v[15 - t]);
+    a[t] = tk ? m : v[t];
+  });
+  /* the diagonal entry, and the row's entries from the diagonal on set to
+   * zero (no lane masks live through the loop below) */
+  float dg = a[0];
+#pragma unroll
+  for (int t = 1; t < 16; ++t)
+    dg = (io == t) ? a[t] : dg;
   if (lane < 16) {
     float u = v[0];
 #pragma unroll
     for (int c = 1; c < 16; ++c)
-      u = (i == c) ? v[c] : u;
+      u = (io == c) ? v[c] : u;
     piv[i] = fabsf(u);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      lds_store4(LU + i * TR + 4 * q, f32x4{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]});
-      /* Urev[r][t] = U[15 - r][15 - t]: this row reversed into row 15 - i */
-      lds_store4(Urev + (15 - i) * TR + 4 * q,
-                 f32x4{v[15 - 4 * q], v[14 - 4 * q], v[13 - 4 * q], v[12 - 4 * q]});
-    }
   }
-  wave_lds_sync();
-  /* lanes 0-15: column c of L^-1 (forward substitution with the unit lower
-   * L); lanes 16-31: column c of T^-1 for the lower triangular T = J U J
-   * = Urev (J the reversal), whose inverse is J U^-1 J.  The same
-   * instruction stream on per-lane rows. */
-  const bool upper = (lane & 16) != 0;
-  const lds_f *T = upper ? Urev : LU;
-  float x[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    /* row r of T four entries at a time (few live registers) */
-    float s = 0.f, dgv = 1.f;
+  for (int t = 0; t < 16; ++t)
+    a[t] = (t < io) ? a[t] : 0.f;
+  const float rd = tk ? __builtin_amdgcn_rcpf(dg) : 1.f;
+  const int h = g >> 1;
+  float x[8]; /* columns 2 s + h of the lane's row */
 #pragma unroll
-    for (int q = 0; q <= r / 4; ++q) {
-      const f32x4 t = lds_load4(T + r * TR + 4 * q);
+  for (int s2 = 0; s2 < 8; ++s2)
+    x[s2] = (2 * s2 + h == io) ? 1.f : 0.f;
+  static_for<15>([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    const float mm = a[c] * dpp_f<0x150 + c>(rd);
+    /* row c is zero beyond column c */
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (4 * q + e < r)
-          s = fmaf(t[e], x[4 * q + e], s);
-        else if (4 * q + e == r)
-          dgv = t[e];
-      }
-    }
-    const float rdg = upper ? __builtin_amdgcn_rcpf(dgv) : 1.f;
-    x[r] = (r == i) ? rdg : (r < i ? 0.f : -s * rdg);
-  }
-  if (lane < 16) {
+    for (int s2 = 0; s2 <= c / 2; ++s2)
+      x[s2] = fmaf(-mm, dpp_f<0x150 + c>(x[s2]), x[s2]);
+  });
+  if (!tk) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r)
-      Linv[r * TR + i] = x[r];
-  } else if (lane < 32) {
-    /* U^-1[15 - r][15 - i] = x[r]: row 15 - i of UinvT, reversed */
+    for (int s2 = 0; s2 < 8; ++s2)
+      Linv[i * TR + 2 * s2 + h] = x[s2];
+  } else {
+    /* T^-1[i][t] = U^-1[15 - i][15 - t] = UinvT[15 - t][15 - i] */
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      lds_store4(UinvT + (15 - i) * TR + 4 * q,
-                 f32x4{x[15 - 4 * q], x[14 - 4 * q], x[13 - 4 * q], x[12 - 4 * q]});
+    for (int s2 = 0; s2 < 8; ++s2)
+      UinvT[(15 - 2 * s2 - h) * TR + 15 - i] = x[s2] * rd;
   }
 }
 
@@ -247,14 +268,6 @@ typedef __attribute__((address_space(3))) int lds_i;
 
 /* f(std::integral_constant<int, i>) for i = 0 .. N - 1, each body inlined
  * separately (constant register indices however long the loop). */
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
-  static_for_seq(f, std::make_integer_sequence<int, N>{});
-}
-
 /* Largest (value, lowest index on ties) over the wavefront, by DPP: xor 1
  * and 2 within quads, the half-row and row mirrors, then rows 0 / 2 into
  * rows 1 / 3 and row 1 into rows 2 / 3; lane 63 ends with the result. */
@@ -455,7 +468,6 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_lu_rs(
   float *LUb = reinterpret_cast<float *>(smem + a.lu_base);
   float *Linvb = reinterpret_cast<float *>(smem + a.linv_base);
   float *UinvTb = reinterpret_cast<float *>(smem + a.uinvt_base);
-  float *Urevb = reinterpret_cast<float *>(smem + a.urev_base);
   float *pivb = reinterpret_cast<float *>(smem + a.piv_base); /* |u_kk| of the dense pivots */
   float *stage = reinterpret_cast<float *>(smem + a.stage_base);
   /* per-lane bases: a tile of wavefront-local index (ia, jb) sits at a
@@ -586,8 +598,8 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_lu_rs(
         }
         if (!PIV && wr == k % WR) {
           wave_lds_sync();
-          factor_diag_tile((const lds_f *)(Xb + k * TSF), (lds_f *)LUb, (lds_f *)Urevb,
-                           (lds_f *)Linvb, (lds_f *)UinvTb, (lds_f *)(pivb + 16 * k), lane, bad);
+          factor_diag_tile((const lds_f *)(Xb + k * TSF), (lds_f *)Linvb, (lds_f *)UinvTb,
+                           (lds_f *)(pivb + 16 * k), lane, bad);
           if (bad)
             *flagA = 1;
         }
