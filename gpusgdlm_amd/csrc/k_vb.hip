@@ -159,10 +159,9 @@ __device__ __forceinline__ void mom_body(long N, int m, int max_p, int chunks,
                                          const unsigned *p, const int *off, long P,
                                          const double *w, const double *lam,
                                          const double *theta, const double *m_t,
-                                         double *part1, double *part2, int j, int c) {
+                                         double *part1, double *part2, int j, int c,
+                                         double (*tiles)[256], double *red) {
   constexpr int NT = NBLK * (NBLK + 1) / 2;
-  __shared__ double tiles[NT][256];
-  __shared__ double red[4];
   const long i0 = N * c / chunks, i1 = N * (c + 1) / chunks;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -243,14 +242,41 @@ __device__ __forceinline__ void mom_body(long N, int m, int max_p, int chunks,
     }
 }
 
+/* mom_body with the fewest 16-wide blocks that hold series j's p_j + 1
+ * columns (at most NBLK): the blocks beyond them would only accumulate
+ * exact zeros and write nothing, so the moments are the same bit for bit
+ * and a series with p_j < 16 runs one MFMA per step instead of three (or
+ * more).  nb is uniform over the workgroup. */
+template <int NBLK>
+__device__ __forceinline__ void mom_series(int nb, long N, int m, int max_p, int chunks,
+                                           const unsigned *p, const int *off, long P,
+                                           const double *w, const double *lam,
+                                           const double *theta, const double *m_t,
+                                           double *part1, double *part2, int j, int c,
+                                           double (*tiles)[256], double *red) {
+  if constexpr (NBLK > 1) {
+    if (nb < NBLK) {
+      mom_series<NBLK - 1>(nb, N, m, max_p, chunks, p, off, P, w, lam, theta, m_t, part1, part2,
+                           j, c, tiles, red);
+      return;
+    }
+  }
+  mom_body<NBLK>(N, m, max_p, chunks, p, off, P, w, lam, theta, m_t, part1, part2, j, c, tiles,
+                 red);
+}
+
 template <int NBLK>
 __global__ __launch_bounds__(256) void k_mom(long N, int m, int max_p, int chunks,
                                              const unsigned *p, const int *off, long P,
                                              const double *w, const double *lam,
                                              const double *theta, const double *m_t,
                                              double *part1, double *part2) {
-  mom_body<NBLK>(N, m, max_p, chunks, p, off, P, w, lam, theta, m_t, part1, part2,
-                 (int)blockIdx.y, (int)blockIdx.x);
+  __shared__ double tiles[NBLK * (NBLK + 1) / 2][256];
+  __shared__ double red[4];
+  const int j = (int)blockIdx.y;
+  const int nb = ((int)p[j] + 1 + 15) / 16;
+  mom_series<NBLK>(nb, N, m, max_p, chunks, p, off, P, w, lam, theta, m_t, part1, part2, j,
+                   (int)blockIdx.x, tiles, red);
 }
 
 /* k_mom of several models: blockIdx.y = series of all models, blockIdx.x =
@@ -260,11 +286,15 @@ __global__ __launch_bounds__(256) void k_mom(long N, int m, int max_p, int chunk
 template <int NBLK>
 __global__ __launch_bounds__(256) void k_mom_grp(const VbModel *__restrict__ tab,
                                                  const int *__restrict__ ser_model) {
+  __shared__ double tiles[NBLK * (NBLK + 1) / 2][256];
+  __shared__ double red[4];
   const VbModel &md = tab[ser_model[blockIdx.y]];
   if ((int)blockIdx.x >= md.chunks)
     return;
-  mom_body<NBLK>(md.N, md.m, md.max_p, md.chunks, md.p, md.off, md.P, md.w, md.lam, md.theta,
-                 md.m_t, md.part1, md.part2, (int)blockIdx.y - md.ser0, (int)blockIdx.x);
+  const int j = (int)blockIdx.y - md.ser0;
+  const int nb = ((int)md.p[j] + 1 + 15) / 16;
+  mom_series<NBLK>(nb, md.N, md.m, md.max_p, md.chunks, md.p, md.off, md.P, md.w, md.lam,
+                   md.theta, md.m_t, md.part1, md.part2, j, (int)blockIdx.x, tiles, red);
 }
 
 __device__ double psi_series(double x) {
