@@ -1885,15 +1885,29 @@ int lu_plan(const Structure &S, int prec, int solve, int n_cu, size_t lds_max,
     plan->rs_flag_base = S.e_rb_flag_base;
     if (solve) {
       /* forecast solves: factors of up to rs_chunk draws in the workspace
-       * (at most 4 GB; a whole number of the factor kernel's rounds) */
+       * (at most 8 GB), a whole number of units that are whole rounds of
+       * both the factor kernel's grid and the substitution kernel's
+       * wavefronts (one draw each), so that no chunk but the last leaves
+       * either launch a partial round; 4 GB and the factor kernel's round
+       * alone when such a unit is larger */
       plan->rs_sol_stride = rs_solve_stride(S, plan->rs_MT);
-      const size_t rounds = std::max<size_t>(
-          1, ((size_t)4 << 30) / (plan->rs_sol_stride * 4) / (size_t)plan->rs_grid);
-      plan->rs_chunk = (int)(std::min<size_t>(rounds, 64) * (size_t)plan->rs_grid);
       if (lu_subst_shape(S, plan->rs_MT, n_cu, &plan->subst_grid, &plan->subst_lds) ||
           plan->subst_lds > lds_max) {
         plan->rs_MT = 0;
         plan->rs_chunk = 0;
+      } else {
+        const size_t per_draw = plan->rs_sol_stride * 4, g = (size_t)plan->rs_grid;
+        const size_t sw = (size_t)plan->subst_grid * 4;
+        size_t unit = g;
+        while (unit % sw)
+          unit += g;
+        size_t n_units = ((size_t)8 << 30) / per_draw / unit;
+        if (n_units == 0) {
+          unit = g;
+          n_units = std::max<size_t>(1, std::min<size_t>(((size_t)4 << 30) / per_draw / g, 64));
+        }
+        plan->rs_unit = (int)unit;
+        plan->rs_chunk = (int)(n_units * unit);
       }
     }
   }
@@ -2085,9 +2099,9 @@ int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,
     int *total = flags + N + 1;
     if (int_op(total, nullptr, 0, st))
       return SGDLM_ERR_DEVICE;
-    /* equal chunks, each a whole number of the factor kernel's rounds */
+    /* equal chunks, each a whole number of units */
     const long nch = (N + plan.rs_chunk - 1) / plan.rs_chunk;
-    const long g = std::max(1, plan.rs_grid);
+    const long g = std::max(1, plan.rs_unit);
     const long per = std::min<long>(plan.rs_chunk, ((N + nch - 1) / nch + g - 1) / g * g);
     for (long c0 = 0; c0 < N; c0 += per) {
       const long n = std::min<long>(per, N - c0);

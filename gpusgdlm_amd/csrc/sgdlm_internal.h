@@ -198,8 +198,9 @@ struct LuPlan {
   /* forecast solves with the register kernel: draws in chunks of
    * rs_chunk, each draw's factors (rs_sol_stride floats) in the workspace
    * from byte rs_sol_off, then one flag word per chunk draw; the
-   * substitution kernel's launch shape */
-  int rs_chunk = 0, subst_grid = 0;
+   * substitution kernel's launch shape; chunks are whole multiples of
+   * rs_unit draws (whole rounds of both kernels' grids) */
+  int rs_chunk = 0, subst_grid = 0, rs_unit = 0;
   size_t rs_sol_stride = 0, rs_sol_off = 0, subst_lds = 0;
 };
 /* Largest dense Schur-complement size (multiple of 16) for which the
