@@ -2084,7 +2084,15 @@ int launch_lu_logdet(const Structure &S, const LuPlan &plan, long N,
   if (rc)
     return rc;
   set_pivot_layout(a, S, plan);
-  return dispatch_pivot<false>(a, plan, st);
+  if ((rc = dispatch_pivot<false>(a, plan, st)))
+    return rc;
+  /* the register kernel with partial pivoting factored every draw with
+   * row interchanges: all of them count as fallback draws, as for the
+   * partial-pivoting kernel alone (after the launch above has read the
+   * list of the draws it could not factor) */
+  if (plan.rs_MT > 0 && plan.rs_piv && set_int(flags, (int)N, st))
+    return SGDLM_ERR_DEVICE;
+  return SGDLM_OK;
 }
 
 int launch_lu_solve(const Structure &S, const LuPlan &plan, long N,

@@ -445,8 +445,15 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_lu_rs(
   int bid, nbk;
   const RsArgs a = grp_args<GRP>(a0, tab, gm, bid, nbk);
   constexpr unsigned LO = 0xffffu;
-  constexpr int UB = 4; /* Schur blocks in flight per wavefront */
-  constexpr int RR = 8; /* rounds loaded ahead per block        */
+  /* Schur gather: blocks in flight per wavefront, rounds loaded ahead
+   * per block (most blocks of the dense blocks' ELL tables have 2 or 3
+   * rounds; the rounds beyond a block's own count are skipped by a
+   * wavefront-uniform test, so padding costs no LDS reads).  Measured at
+   * m = 400, N = 5e4, FP32 LU: (UB, RR) = (1, 4) 25.8 ms, (1, 2..5)
+   * 25.8-26.0, (2, 4) 26.7, (4, 3) 27.3, (4, 8) with the padding read
+   * 28.1, (8, 3) 33.3 (spills) */
+  constexpr int UB = 1;
+  constexpr int RR = 4;
   const int tid = threadIdx.x, lane = tid & 63;
   /* wavefront-uniform (scalar) values: tile ownership tests compile to
    * scalar branches */
@@ -543,7 +550,8 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_lu_rs(
           float s = 0.f;
 #pragma unroll
           for (int r = 0; r < RR; ++r)
-            s = fmaf(-sl[pv[u][r] & LO], sl[pv[u][r] >> 16], s);
+            if (r < R[u])
+              s = fmaf(-sl[pv[u][r] & LO], sl[pv[u][r] >> 16], s);
           for (int rb = RR; rb < R[u]; rb += RR) {
             unsigned c[RR];
 #pragma unroll
@@ -551,7 +559,8 @@ __global__ __launch_bounds__(64 * WR * WC, 2) void k_lu_rs(
               c[r] = rb + r < R[u] ? a.st_pairs[(size_t)cb[u] + 64 * (rb + r) + lane] : a.zero_pair;
 #pragma unroll
             for (int r = 0; r < RR; ++r)
-              s = fmaf(-sl[c[r] & LO], sl[c[r] >> 16], s);
+              if (rb + r < R[u])
+                s = fmaf(-sl[c[r] & LO], sl[c[r] >> 16], s);
           }
           if (off[u] >= 0)
             stage[off[u]] = s;

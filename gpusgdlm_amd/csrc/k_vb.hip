@@ -454,11 +454,20 @@ __global__ __launch_bounds__(64) void k_vb_final_grp(const VbModel *__restrict__
 }
 
 /* After it: the weight statistics of every model side by side, each
- * followed by its LU fallback count (one copy to the host). */
+ * followed by its LU fallback count (one copy to the host); a model whose
+ * count is fixed after the LU (flag_final) gets it stored first. */
 __global__ void k_grp_stats(const VbModel *__restrict__ tab, int n, double *out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 5 * n)
-    out[i] = i % 5 < 4 ? tab[i / 5].stats[i % 5] : (double)tab[i / 5].lu_flags[0];
+  if (i < 5 * n) {
+    const VbModel &md = tab[i / 5];
+    if (i % 5 < 4) {
+      out[i] = md.stats[i % 5];
+    } else {
+      if (md.flag_final >= 0)
+        md.lu_flags[0] = md.flag_final;
+      out[i] = (double)md.lu_flags[0];
+    }
+  }
 }
 
 } // namespace

@@ -1286,6 +1286,7 @@ int build_vb_group(sgdlm_ctx *const *cs, int n, const std::vector<VbDev> &Ds,
                       tuned ? c->tuned.draw_wg_ms : 0.0, &init, &k1, &k2))
       return give_up(SGDLM_ERR_ARG);
     tab[i].flag_init = init;
+    tab[i].flag_final = plans[i].rs_MT > 0 && plans[i].rs_piv ? (int)tab[i].N : -1;
     auto add = [&](std::vector<std::vector<Item>> &st, std::map<const void *, size_t> &at,
                    LuGrpKernel &k) {
       auto it = at.find(k.fn);

@@ -106,6 +106,9 @@ struct VbModel {
   double *U = nullptr, *lam = nullptr, *theta = nullptr, *logdet = nullptr;
   int *lu_flags = nullptr;       /* flags[0] = flag_init before the LU      */
   int flag_init = 0;
+  /* >= 0: flags[0] after the LU (the register kernel with partial
+   * pivoting factors every draw with row interchanges: all N count) */
+  int flag_final = -1;
   /* VbBuffers of the model */
   double *w = nullptr, *stats = nullptr, *part1 = nullptr, *part2 = nullptr;
   double *mean_lambda = nullptr, *mean_log_lambda = nullptr, *mean_m = nullptr,

@@ -226,8 +226,8 @@ int sgdlm_set_timing(sgdlm_ctx *ctx, int enable);
 int sgdlm_get_timing(const sgdlm_ctx *ctx, const char *stage, double *ms,
                      long *launches);
 /* Number of draws of the last VB or forecast call whose LU needed row
- * interchanges (factored by the partial-pivoting kernel instead of the fast
- * path). */
+ * interchanges (factored by a partial-pivoting kernel instead of the fast
+ * path; every draw when the call's kernel kind pivots every draw). */
 int sgdlm_get_lu_fallback_count(const sgdlm_ctx *ctx, long *count);
 /* Workgroups / LDS bytes / panel width / workgroup size of the fast VB LU
  * kernel for the current structure and precision (nb = 0: every draw goes
