@@ -226,7 +226,9 @@ __global__ __launch_bounds__(256, 1) void k_lu_wsch(WschArgs a0, const WschArgs 
                                                    GrpMap gm) {
   constexpr int R = (M + 63) / 64;
   constexpr unsigned LO = 0xffffu;
-  constexpr int RR = 8; /* pair rounds loaded ahead */
+  /* pair rounds loaded ahead; a block's rounds beyond its own count are
+   * skipped by a wavefront-uniform test (no zero-pair LDS reads) */
+  constexpr int RR = 4;
   typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type V16;
   constexpr int VW = 16 / sizeof(T);
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(256, 1) void k_lu_wsch(WschArgs a0, const WschArgs 
               pv[r] = r0 + r < nr ? a.st_pairs[(size_t)cb + 64 * (r0 + r) + lane] : a.zero_pair;
 #pragma unroll
             for (int r = 0; r < RR; ++r)
-              acc = fma(-sl[pv[r] & LO], sl[pv[r] >> 16], acc);
+              if (r0 + r < nr)
+                acc = fma(-sl[pv[r] & LO], sl[pv[r] >> 16], acc);
           }
           if (off >= 0)
             stage[off] = acc;
